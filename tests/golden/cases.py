@@ -1,0 +1,284 @@
+"""Golden-vector case catalogue.
+
+Each builder returns a small ``meshgen.Problem`` exercising one corner of
+the transport path: every light-source type and direction type, mirror and
+Fresnel boundaries, internal index jumps, vacuum (mus=0) and non-absorbing
+(mua=0) regions, isotropic scattering (g=0), frequency-domain tallies and a
+negative phase0.  ``ERROR_CASES`` hold inputs the reference rejects.
+The generator (make_golden.py) runs the reference engine on them and stores
+inputs and outputs; the tests replay the inputs through the oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from valomc_amd import meshgen as mg
+
+
+def _box(nc: int, h: float = 1.0):
+    half = nc * h / 2
+    v = np.arange(-half + h / 2, half - h / 2 + h / 4, h)
+    return mg.grid_mesh(v, v, v)
+
+
+def _faces_on(r, BH, axis, value):
+    tri = r[BH]
+    return np.nonzero(np.all(np.abs(tri[:, :, axis] - value) < 1e-9, axis=1))[0]
+
+
+def p3_pencil_sphere():
+    p = mg.sphere_box_3d(nphoton=20000, seed=11, side_mm=6, pencil=(0.3, 0.2))
+    cen = p.r[p.H].mean(axis=1)
+    blob = np.linalg.norm(cen - np.array([0.5, -0.5, 0.5]), axis=1) < 1.6
+    p.mua[blob], p.mus[blob] = 0.3, 3.0
+    return p
+
+
+def p3_cosinic_hetero_n():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.05, 1.0, 0.8, 1.4, 1.0)
+    cen = r[H].mean(axis=1)
+    core = np.all(np.abs(cen) < 1.0, axis=1)
+    p.n[core], p.mus[core], p.mua[core] = 1.6, 2.0, 0.1
+    top = _faces_on(r, BH, 2, 2.0)
+    p.BCType[top[:6]] = ord("c")
+    p.Nphoton, p.seed = 20000, 5
+    return p
+
+
+def p3_direct_abs_fd():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.02, 1.5, 0.9, 1.37, 1.0)
+    top = _faces_on(r, BH, 2, 2.0)
+    p.BCType[top[2:5]] = ord("l")
+    p.BCLDirType = np.full(p.Nb, ord("n"), dtype=np.uint8)
+    p.BCLDirType[top[2:5]] = ord("a")
+    p.BCLNormal = np.zeros((p.Nb, 3))
+    p.BCLNormal[top[2:5]] = (0.3, 0.1, -1.5)      # absolute, deliberately unnormalised
+    p.f, p.phase0 = 100e6, -1.0
+    p.Nphoton, p.seed = 20000, 9
+    return p
+
+
+def p3_iso_mirror():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.05, 2.0, 0.7, 1.33, 1.0)
+    bottom = _faces_on(r, BH, 2, -2.0)
+    xm = _faces_on(r, BH, 0, -2.0)
+    xp = _faces_on(r, BH, 0, 2.0)
+    p.BCType[bottom[:4]] = ord("i")
+    p.BCType[xm] = ord("m")
+    p.BCType[xp[:8]] = ord("C")
+    p.BCn[_faces_on(r, BH, 1, 2.0)] = 0.0         # no Fresnel on +y
+    p.BCn[_faces_on(r, BH, 1, -2.0)] = 1.5
+    p.Nphoton, p.seed = 15000, 21
+    return p
+
+
+def p3_relative_cos_intensity():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.03, 1.2, 0.85, 1.4, 1.0)
+    yp = _faces_on(r, BH, 1, 2.0)
+    p.BCType[yp[:10]] = ord("c")
+    p.BCLDirType = np.full(p.Nb, ord("n"), dtype=np.uint8)
+    p.BCLDirType[yp[:10]] = ord("r")
+    p.BCLNormal = np.zeros((p.Nb, 3))
+    p.BCLNormal[yp[:10]] = (0.2, 0.1, 1.0)
+    p.BCIntensity = np.linspace(0.5, 2.0, p.Nb)
+    p.Nphoton, p.seed = 20000, 33
+    return p
+
+
+def p3_vacuum_nonabsorbing():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.04, 1.0, 0.9, 1.4, 1.0)
+    cen = r[H].mean(axis=1)
+    p.mus[(cen[:, 2] > 0.0) & (cen[:, 2] < 1.0)] = 0.0   # vacuum slab
+    p.mua[cen[:, 0] > 1.0] = 0.0                          # non-absorbing slab
+    p.g[cen[:, 1] < -1.0] = 0.0                           # isotropic scattering
+    top = _faces_on(r, BH, 2, 2.0)
+    p.BCType[top[:8]] = ord("l")
+    p.Nphoton, p.seed = 20000, 44
+    return p
+
+
+def p3_given_L_I():
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.05, 1.0, 0.9, 1.4, 1.0)
+    top = _faces_on(r, BH, 2, 2.0)
+    bot = _faces_on(r, BH, 2, -2.0)
+    p.BCType[top[:4]] = ord("L")
+    p.BCType[bot[:4]] = ord("I")
+    p.BCLDirType = np.full(p.Nb, ord("n"), dtype=np.uint8)
+    p.BCLDirType[top[:4]] = ord("a")
+    p.BCLDirType[bot[:4]] = ord("a")
+    p.BCLNormal = np.zeros((p.Nb, 3))
+    p.BCLNormal[top[:4]] = (0.0, 0.6, -0.8)
+    p.BCLNormal[bot[:4]] = (0.0, 0.0, 1.0)
+    p.Nphoton, p.seed = 15000, 55
+    return p
+
+
+def p3_single_cube():
+    r, H, BH = mg.grid_mesh([0.5], [0.5], [0.5], h=1.0)   # one cube, 6 tets, 12 faces
+    p = mg.homogeneous(3, r, H, BH, 0.1, 5.0, 0.5, 1.3, 1.0)
+    p.BCType[0] = ord("c")
+    p.Nphoton, p.seed = 10000, 3
+    return p
+
+
+def p2_square_cosinic():
+    return mg.square_2d(nphoton=20000, seed=1)
+
+
+def p2_doc_quad():
+    r = np.array([[-1.0, -1.0], [-1.0, 1.0], [1.0, 1.0], [1.0, -1.0]])
+    H = np.array([[1, 2, 3], [3, 4, 1]], dtype=np.int64) - 1
+    BH = np.array([[1, 2], [2, 3], [3, 4], [4, 1]], dtype=np.int64) - 1
+    p = mg.homogeneous(2, r, H, BH, 0.1, 2.0, 0.8, 1.3, 1.0)
+    p.BCType[0] = ord("l")
+    p.Nphoton, p.seed = 10000, 2
+    return p
+
+
+def p2_iso_gauss_fd():
+    r, H, BH = mg.rectangular_mesh(6, 6, 1)
+    p = mg.homogeneous(2, r, H, BH, 0.05, 1.5, 0.85, 1.35, 1.0)
+    p.BCType[0:3] = ord("i")
+    p.BCType[6:9] = ord("g")
+    p.GaussianSigma = np.full(p.Nb, 0.3)
+    p.f, p.phase0 = 100e6, -0.5
+    p.Nphoton, p.seed = 20000, 8
+    return p
+
+
+def p2_given_directions():
+    r, H, BH = mg.rectangular_mesh(6, 6, 1)
+    p = mg.homogeneous(2, r, H, BH, 0.02, 1.0, 0.9, 1.4, 1.0)
+    p.BCType[0:2] = ord("c")
+    p.BCType[6:8] = ord("g")
+    p.BCType[12:14] = ord("l")
+    p.BCType[18:20] = ord("i")
+    p.GaussianSigma = np.full(p.Nb, 0.5)
+    p.BCLDirType = np.full(p.Nb, ord("n"), dtype=np.uint8)
+    p.BCLDirType[0:2] = ord("a")
+    p.BCLDirType[6:8] = ord("r")
+    p.BCLDirType[12:14] = ord("a")
+    p.BCLDirType[18:20] = ord("r")
+    p.BCLNormal = np.zeros((p.Nb, 2))
+    p.BCLNormal[0:2] = (1.0, 0.7)
+    p.BCLNormal[6:8] = (0.4, 1.0)
+    p.BCLNormal[12:14] = (0.5, -0.5)
+    p.BCLNormal[18:20] = (-0.3, 1.0)
+    p.BCIntensity = np.linspace(1.0, 3.0, p.Nb)
+    p.Nphoton, p.seed = 20000, 12
+    return p
+
+
+def p2_mirror_hetero():
+    r, H, BH = mg.rectangular_mesh(6, 6, 1)
+    p = mg.homogeneous(2, r, H, BH, 0.05, 1.0, 0.9, 1.4, 1.0)
+    cen = r[H].mean(axis=1)
+    p.n[np.all(np.abs(cen) < 1.5, axis=1)] = 1.7
+    p.mus[(cen[:, 0] > 1.5)] = 0.0
+    p.mua[(cen[:, 1] > 1.5)] = 0.0
+    p.g[(cen[:, 1] < -1.5)] = 0.0
+    p.BCType[0:6] = ord("c")
+    p.BCType[6:12] = ord("m")
+    p.BCn[12:18] = 0.0
+    p.Nphoton, p.seed = 20000, 77
+    return p
+
+
+CASES = {
+    "p3_pencil_sphere": p3_pencil_sphere,
+    "p3_cosinic_hetero_n": p3_cosinic_hetero_n,
+    "p3_direct_abs_fd": p3_direct_abs_fd,
+    "p3_iso_mirror": p3_iso_mirror,
+    "p3_relative_cos_intensity": p3_relative_cos_intensity,
+    "p3_vacuum_nonabsorbing": p3_vacuum_nonabsorbing,
+    "p3_given_L_I": p3_given_L_I,
+    "p3_single_cube": p3_single_cube,
+    "p2_square_cosinic": p2_square_cosinic,
+    "p2_doc_quad": p2_doc_quad,
+    "p2_iso_gauss_fd": p2_iso_gauss_fd,
+    "p2_given_directions": p2_given_directions,
+    "p2_mirror_hetero": p2_mirror_hetero,
+}
+
+THREADS = (1, 4)
+
+
+def _err_no_source():
+    p = p2_doc_quad()
+    p.BCType[:] = ord("a")
+    return p
+
+
+def _err_bad_h():
+    p = p3_single_cube()
+    p.H = p.H.copy()
+    p.H[2, 1] = p.Nr + 3
+    return p
+
+
+def _err_dup_neighbour():
+    # a third tet sharing face (v0, v1, v2) of the first two
+    r = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -1], [0.3, 0.3, 1.5]], dtype=float)
+    H = np.array([[0, 1, 2, 3], [0, 1, 2, 4], [0, 1, 2, 5]], dtype=np.int64)
+    BH = np.array([[0, 1, 3], [0, 2, 3], [1, 2, 3]], dtype=np.int64)
+    p = mg.homogeneous(3, r, H, BH, 0.1, 1.0, 0.9, 1.4, 1.0)
+    p.BCType[0] = ord("c")
+    p.Nphoton = 10
+    return p
+
+
+def _err_missing_boundary_2d():
+    p = p2_doc_quad()
+    p.BH = p.BH[:3].copy()
+    p.BCType, p.BCn = p.BCType[:3].copy(), p.BCn[:3].copy()
+    return p
+
+
+def _err_gauss_without_sigma():
+    p = p2_doc_quad()
+    p.BCType[1] = ord("g")
+    return p
+
+
+ERROR_CASES = {
+    "err_no_source_2d": _err_no_source,
+    "err_bad_h_3d": _err_bad_h,
+    "err_duplicate_neighbour_3d": _err_dup_neighbour,
+    "err_missing_boundary_2d": _err_missing_boundary_2d,
+    "err_gauss_without_sigma_2d": _err_gauss_without_sigma,
+}
+
+
+INPUT_KEYS = ("H", "BH", "r", "mua", "mus", "g", "n", "BCType", "BCn", "BCLNormal",
+              "BCLDirType", "BCIntensity", "GaussianSigma", "HN")
+SCALAR_KEYS = ("dim", "f", "phase0", "Nphoton", "seed")
+
+
+def to_arrays(p) -> dict:
+    d = {}
+    for k in INPUT_KEYS:
+        v = getattr(p, k)
+        if v is not None:
+            d["in_" + k] = np.asarray(v)
+    for k in SCALAR_KEYS:
+        d["in_" + k] = np.asarray(getattr(p, k))
+    return d
+
+
+def from_arrays(d) -> "mg.Problem":
+    kw = {}
+    for k in INPUT_KEYS:
+        kw[k] = np.asarray(d["in_" + k]) if ("in_" + k) in d else None
+    p = mg.Problem(dim=int(d["in_dim"]), **{k: kw[k] for k in ("H", "BH", "r", "mua", "mus", "g", "n",
+                                                                  "BCType", "BCn")})
+    for k in ("BCLNormal", "BCLDirType", "BCIntensity", "GaussianSigma", "HN"):
+        setattr(p, k, kw[k])
+    p.f, p.phase0 = float(d["in_f"]), float(d["in_phase0"])
+    p.Nphoton, p.seed = int(d["in_Nphoton"]), int(d["in_seed"])
+    return p

@@ -1,0 +1,124 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+* golden vectors written by the reference engine itself (tests/golden/):
+  the MT19937 mode must reproduce every output bit at OpenMP thread counts
+  1 and 4, plus the neighbour table, the loss count and the error statuses;
+* known-answer vectors for both random streams;
+* the energy ledger: absorbed + exited + roulette killed - roulette gained
+  + lost == launched photons.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import pyoracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+OUT = ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")
+
+
+def _golden(name):
+    return dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+@pytest.mark.parametrize("threads", cases.THREADS)
+def test_oracle_reproduces_reference_bits(name, threads):
+    d = _golden(name)
+    p = cases.from_arrays(d)
+    res = pyoracle.run(p, threads, pyoracle.RNG_MT19937)
+    for k in OUT:
+        ref = d[f"out_T{threads}_{k}"]
+        got = getattr(res, k)
+        assert np.array_equal(got, ref), (k, np.max(np.abs(got - ref)))
+    assert res.loss == int(d[f"out_T{threads}_loss"])
+    nv = p.dim + 1
+    hn_ref = d[f"out_T{threads}_HN"].reshape(-1, nv) if d[f"out_T{threads}_HN"].ndim == 1 \
+        else d[f"out_T{threads}_HN"]
+    assert np.array_equal(res.HN, hn_ref)
+
+
+@pytest.mark.parametrize("name", sorted(cases.ERROR_CASES))
+def test_oracle_error_status_matches_reference(name):
+    d = _golden(name)
+    p = cases.from_arrays(d)
+    with pytest.raises(pyoracle.OracleError) as ei:
+        pyoracle.run(p, 1)
+    assert ei.value.status == int(d["out_status"])
+
+
+def test_golden_set_is_complete():
+    files = {os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz"))}
+    assert set(cases.CASES) | set(cases.ERROR_CASES) <= files
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32_10
+    kat = [([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+           ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+           ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+            [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1])]
+    for ctr, key, expect in kat:
+        assert pyoracle.philox(ctr, key) == expect
+
+
+def test_mt19937_known_answer():
+    # mt_rng == MT19937 with init_genrand seeding (cpp/3d/mt_rng.hpp:119-125);
+    # the C++ standard fixes the 10000th output for seed 5489.
+    rs = np.random.RandomState(5489)
+    assert int(rs._bit_generator.random_raw(10000)[-1]) == 4123659995
+
+
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_iso_mirror", "p3_vacuum_nonabsorbing",
+                                  "p2_mirror_hetero", "p2_iso_gauss_fd"])
+@pytest.mark.parametrize("mode", [pyoracle.RNG_MT19937, pyoracle.RNG_PHILOX])
+def test_energy_ledger_closes(name, mode):
+    p = cases.from_arrays(_golden(name))
+    res = pyoracle.run(p, 2, mode)
+    absorbed, exited, killed, gained, lost = res.ledger
+    total = absorbed + exited + killed - gained + lost
+    assert abs(total - p.Nphoton) <= 1e-9 * p.Nphoton
+    # lost photons carry their weight into the ledger
+    if res.loss == 0:
+        assert lost == 0.0
+
+
+def test_ledger_matches_tallies_cw():
+    """CW, mua > 0 everywhere: absorbed weight equals sum(ER * mua * V) * N
+    and exited weight equals sum(EBR * A) * N (MC3D.hpp:1793-1804)."""
+    p = cases.from_arrays(_golden("p3_pencil_sphere"))
+    res = pyoracle.run(p, 1)
+    r, H, BH = p.r, p.H, p.BH
+    a, b, c, d = (r[H[:, i]] for i in range(4))
+    V = np.abs(np.einsum("ij,ij->i", a - d, np.cross(b - d, c - d))) / 6.0
+    t = r[BH]
+    A = 0.5 * np.linalg.norm(np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]), axis=1)
+    N = p.Nphoton
+    assert np.isclose(np.sum(res.ER * p.mua * V) * N, res.ledger[0], rtol=1e-10)
+    assert np.isclose(np.sum(res.EBR * A) * N, res.ledger[1], rtol=1e-10)
+
+
+def test_philox_mode_statistically_matches_mt():
+    """Two stream families, same physics: fluence agrees within noise."""
+    p = cases.from_arrays(_golden("p2_square_cosinic"))
+    p.Nphoton = 200000
+    a = pyoracle.run(p, 4, pyoracle.RNG_MT19937)
+    b = pyoracle.run(p, 4, pyoracle.RNG_PHILOX)
+    p.seed += 100
+    c = pyoracle.run(p, 4, pyoracle.RNG_MT19937)
+    rel = np.linalg.norm(a.ER - b.ER) / np.linalg.norm(a.ER)
+    spread = np.linalg.norm(a.ER - c.ER) / np.linalg.norm(a.ER)
+    assert rel < 3 * spread + 1e-3
+
+
+@pytest.mark.skipif(not pyoracle.have_reference(), reason="reference engine not built here")
+def test_live_reference_matches_oracle_other_seed():
+    p = cases.p3_cosinic_hetero_n()
+    p.seed, p.Nphoton = 123456789, 7001
+    a = pyoracle.run(p, 3)
+    b = pyoracle.run_reference(p, 3)
+    for k in OUT:
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
