@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Benchmark: photons/s of the MI355X transport on the 384k-tet box.
+
+Workload (one "step" = one MonteCarlo call): 40x40x40 mm box, 1 mm grid,
+6 tets per cube (384,000 tets, 68,921 nodes); background mua=0.01/mm,
+mus=1/mm, g=0.9, n=1.4 (n=1.0 outside); a 5 mm sphere (mua=0.05, mus=2) at
+a seeded position; pencil source at (0.3, 0.2, 20) on the top face; CW;
+1e8 photons per step.  The timed path is FP32 transport with FP64 tallies;
+an untimed FP64 run on the same photon streams gives the accuracy check
+(relative L2 of element fluence) and the FP64 throughput.
+
+Prints one JSON line.  `value` is photons/s over the timed steps with the
+mesh already resident in HBM.  `roofline` uses the walk kernel's duration
+from HIP events on its own stream; `cpu_baseline` times the reference
+engine (compiled from its sources into oracle/_ref/) with OpenMP on the host
+cores over a bounded photon sample of the same problem.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photons/s at <=1% fluence L2 err, batched FP32 vs plain FP64; p99 latency"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per element step: face planes + neighbour codes +
+# optical parameters of the element, plus the tally read-modify-write
+B_STEP = {("fp32", False): 112, ("fp32", True): 128, ("fp64", False): 192, ("fp64", True): 208}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--photons", type=float, default=1e8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--block-size", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--refill-min", type=int, default=0)
+    ap.add_argument("--replicas", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(p, target_s: float):
+    """Time the reference engine (or, without it, the oracle) on the host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    kind = "reference" if pyoracle.have_reference() else "port"
+    run = pyoracle.run_reference if kind == "reference" else pyoracle.run
+    # calibrate on a short run, then size the sample to ~target_s
+    p.Nphoton = 20000
+    r = run(p, threads)
+    rate = p.Nphoton / max(r.seconds, 1e-6)
+    n = int(min(max(rate * target_s, 20000), 5e7))
+    p.Nphoton = n
+    r = run(p, threads)
+    return {"value": n / r.seconds, "unit": "photons/s", "cores": threads, "kind": kind,
+            "sample": f"{n} photons of the same 384k-tet problem, MonteCarlo() only, "
+                      f"{threads} OpenMP threads, {r.seconds:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is a "
+              "design note, not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+    import valomc_amd as vmc
+    from valomc_amd import meshgen
+
+    nphot = int(a.photons)
+    p = meshgen.sphere_box_3d(nphoton=nphot, seed=1234)
+    prob = vmc.Problem(p)
+    kw = dict(block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, refill_min=a.refill_min,
+              replicas=a.replicas)
+
+    for i in range(a.warmup):
+        prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)
+    runs = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        runs.append(prob.run(nphot, seed=1 + i, precision=vmc.FP32, **kw))
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = a.steps * nphot / wall
+    kernel_s = sum(r.kernel_ms for r in runs) / 1e3
+    steps_total = sum(r.steps for r in runs)
+    achieved = steps_total * B_STEP[("fp32", False)] / kernel_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "walk_kernel<3,float,CW,fp64-tally>",
+                "bytes_per_step": B_STEP[("fp32", False)],
+                "steps_per_launch": steps_total // a.steps,
+                "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
+
+    extra = {}
+    if not a.no_fp64:
+        f32 = runs[0]
+        t = time.perf_counter()
+        f64 = prob.run(nphot, seed=1, precision=vmc.FP64, **kw)
+        t64 = time.perf_counter() - t
+        rel = float(np.linalg.norm(f32.ER - f64.ER) / np.linalg.norm(f64.ER))
+        extra = {"fp64_photons_per_s": round(nphot / t64, 1),
+                 "fp32_speedup_vs_fp64": round((nphot / (wall / a.steps)) / (nphot / t64), 3),
+                 "fp32_vs_fp64_rel_l2": rel,
+                 "fp64_kernel_ms": round(f64.kernel_ms, 3)}
+    prob.close()
+
+    led = runs[-1].ledger
+    balance = abs(led[0] + led[1] + led[2] - led[3] + led[4] - nphot) / nphot
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(meshgen.sphere_box_3d(nphoton=1, seed=1234), a.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "384k-tet heterogeneous box, pencil source, CW, FP32 transport + FP64 tallies",
+                   "elements": p.Ne, "boundary_faces": p.Nb, "nodes": p.Nr,
+                   "photons_per_step": nphot, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "element_steps_per_s": round(steps_total / kernel_s, 1),
+        "energy_balance_rel": balance,
+        "loss_per_step": runs[-1].loss,
+        "grid": [runs[-1].grid_blocks, runs[-1].block_size],
+        **extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

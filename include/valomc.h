@@ -1,0 +1,164 @@
+/*
+ * valomc.h -- C ABI of the MI355X photon-transport engine.
+ *
+ * Drop-in boundary for ValoMC's MC3D/MC2D::MonteCarlo.  Inputs are the
+ * arrays the reference engine holds as public members (cpp/3d/MC3D.hpp:108-172,
+ * cpp/2d/MC2D.hpp:113-180) in the reference's own layout: column-major
+ * (Fortran order, cpp/3d/Array.hpp:58-66), 0-based int64 node indices,
+ * boundary neighbours encoded as -1-ib.  Outputs are the normalised
+ * ER/EI/EBR/EBI/DEBR/DEBI the reference leaves in those members after
+ * MonteCarlo() (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703).
+ *
+ * No exceptions cross this boundary.  Status codes:
+ *   0            success
+ *   1 .. 16      1 + the reference's mcerror code (cpp/Errors.hpp:4-21)
+ *   17           a light source whose direction the reference leaves
+ *                undefined ('L', 'I' or 'p' with direction type 'n',
+ *                MC3D.hpp:1112-1192 has no branch for them)
+ *   18           invalid argument (NULL pointer, bad size or option)
+ *   100 ..       device errors (see vmc_status_string / vmc_last_error)
+ *
+ * Threading: one problem handle per host thread; calls on one handle are
+ * serialised by the caller.  Device memory and streams are owned by the
+ * library and released by vmc_problem_destroy.
+ */
+#ifndef VALOMC_H
+#define VALOMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VMC_OK 0
+#define VMC_ERR_UNDEFINED_DIRECTION 17
+#define VMC_ERR_INVALID_ARGUMENT 18
+#define VMC_ERR_HIP 100
+#define VMC_ERR_OUT_OF_MEMORY 101
+#define VMC_ERR_NO_DEVICE 102
+#define VMC_ERR_ABORTED 103
+
+/* transport arithmetic */
+#define VMC_FP64 0 /* plain double-precision transport (the GPU FP64 oracle)   */
+#define VMC_FP32 1 /* single-precision transport, FP64 tallies                 */
+
+/* tally accumulation */
+#define VMC_TALLY_FP64 0    /* double-precision atomics (order-dependent rounding) */
+#define VMC_TALLY_FIXED64 1 /* 64-bit fixed point, 2^-32 resolution: bit-reproducible */
+
+/* Geometry, medium and boundary conditions: everything MC3D/MC2D holds
+   before Init().  dim = 3 for tetrahedra (MC3D), 2 for triangles (MC2D). */
+typedef struct vmc_problem_desc {
+  int32_t dim;
+  int64_t Ne, Nb, Nr;
+  const int64_t *H;                  /* [Ne*(dim+1)]                        */
+  const int64_t *HN;                 /* optional [Ne*(dim+1)], NULL = build  */
+  const int64_t *BH;                 /* [Nb*dim]                             */
+  const double *r;                   /* [Nr*dim]                             */
+  const double *mua, *mus, *g, *n;   /* [Ne]                                 */
+  const char *BCType;                /* [Nb] a m l L i I c C p (3D), a m l i c g L I C (2D) */
+  const double *BCn;                 /* [Nb] exterior index; <= 0: no Fresnel */
+  const double *BCLNormal;           /* optional [Nb*dim]                    */
+  const char *BCLightDirectionType;  /* optional [Nb] n a r p                */
+  const double *BCIntensity;         /* optional [Nb]                        */
+  const double *GaussianSigma;       /* optional [Nb], 2D 'g' sources        */
+} vmc_problem_desc;
+
+/* One forward solve. */
+typedef struct vmc_run_desc {
+  double f;              /* modulation frequency [Hz]; 0 = CW                  */
+  double phase0;         /* initial phase; negatives wrap into [0, 2pi)        */
+  int64_t Nphoton;
+  uint64_t seed;         /* photon i of a run draws from the stream (seed, i)  */
+  int32_t precision;     /* VMC_FP64 | VMC_FP32                                */
+  int32_t tally;         /* VMC_TALLY_FP64 | VMC_TALLY_FIXED64                 */
+  int32_t block_size;    /* 0 = default (256)                                  */
+  int32_t blocks_per_cu; /* 0 = as many as fit                                 */
+  int32_t replicas;      /* tally copies, 0 = one per XCD (8)                  */
+  int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 1 */
+  uint32_t stream_tag;   /* extra stream word: runs with different tags are
+                            independent (batch instances use their index)      */
+  int32_t reserved[5];
+} vmc_run_desc;
+
+typedef struct vmc_output {
+  double *ER, *EI;       /* [Ne]; may be NULL                                   */
+  double *EBR, *EBI;     /* [Nb]                                                */
+  double *DEBR, *DEBI;   /* [Nb] (2D only; zero in 3D like the reference)       */
+} vmc_output;
+
+typedef struct vmc_stats {
+  int64_t photons;       /* photons launched                                    */
+  int64_t loss;          /* MC3D::loss: photons that left the mesh topology     */
+  int64_t steps;         /* element steps executed on the device                */
+  double kernel_ms;      /* transport kernels, HIP events on their stream       */
+  double total_ms;       /* whole vmc_problem_run call                          */
+  /* raw weight ledger: absorbed, exited, roulette killed, roulette gained,
+     lost.  absorbed + exited + killed - gained + lost == photons.          */
+  double ledger[5];
+  int64_t rechecks;      /* FP32 exit-face choices re-evaluated in FP64          */
+  int32_t grid_blocks, block_size;
+} vmc_stats;
+
+/* progress callback: percent done; return 0 to abort (MC3D.hpp:1688-1695) */
+typedef int (*vmc_progress_fn)(double percent, void *user);
+
+typedef struct vmc_problem vmc_problem;
+
+/* ErrorChecks() + Init() of the reference (MC3D.hpp:460-738): validates,
+   builds the neighbour table and light sources, uploads element records to
+   the device.  The problem stays resident until destroyed. */
+int vmc_problem_create(const vmc_problem_desc *desc, vmc_problem **out);
+
+/* MonteCarlo() (MC3D.hpp:1641-1822): transport Nphoton photons and write
+   normalised tallies.  If progress returns 0 the run stops after the
+   current chunk and, like the reference, still normalises by the planned
+   Nphoton; the status is then VMC_ERR_ABORTED. */
+int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
+                    vmc_stats *stats, vmc_progress_fn progress, void *user);
+
+/* Host-only ErrorChecks() + the host part of Init(): validation, neighbour
+   table and light-source list, without touching a device.  Replaces the
+   reference's use of MC3D::BuildNeighbourhoods by createBH3mex
+   (cpp/3d/createBH3mex.cpp:26-32).  HN_out (optional) receives the
+   neighbour table [Ne*(dim+1)], nsources (optional) the number of
+   light-source faces. */
+int vmc_problem_check(const vmc_problem_desc *desc, int64_t *HN_out, int64_t *nsources);
+
+/* The neighbour table used (reference HN convention), [Ne*(dim+1)]. */
+int vmc_problem_get_hn(const vmc_problem *prob, int64_t *HN);
+
+/* Replace the per-element optical parameters of a resident problem
+   (any pointer may be NULL to keep the current values). */
+int vmc_problem_set_medium(vmc_problem *prob, const double *mua, const double *mus,
+                           const double *g, const double *n);
+
+void vmc_problem_destroy(vmc_problem *prob);
+
+/* One-shot call with the argument list of the MEX gateways
+   (cpp/3d/MC3Dmex.cpp:106-110, cpp/2d/MC2Dmex.cpp:105-109): create, run,
+   destroy.  HN may be NULL; HN_out (optional) receives the table used.
+   GaussianSigma is only read for dim == 2. */
+int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr,
+                    const int64_t *H, const int64_t *HN, const int64_t *BH, const double *r,
+                    const char *BCType, const double *BCIntensity,
+                    const char *BCLightDirectionType, const double *BCLNormal,
+                    const double *BCn, const double *mua, const double *mus,
+                    const double *g, const double *n, double f, double phase0,
+                    int64_t Nphoton, const double *GaussianSigma, uint64_t seed,
+                    int32_t precision, double *ER, double *EI, double *EBR, double *EBI,
+                    double *DEBR, double *DEBI, int64_t *HN_out, int64_t *loss);
+
+const char *vmc_status_string(int status);
+/* last device error text of the calling thread */
+const char *vmc_last_error(void);
+/* device name, CU count, HBM size; returns VMC_ERR_NO_DEVICE without a GPU */
+int vmc_device_info(char *buf, int32_t buflen, int32_t *num_cus);
+/* library version string */
+const char *vmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,133 @@
+"""C ABI checks that need no GPU: the library loads, exports exactly what
+include/valomc.h declares, and its host pre-pass (ErrorChecks + Init
+without the device) reproduces the reference's neighbour tables and error
+statuses from the golden vectors."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import valomc_amd as vmc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "valomc.h")).read()
+    return set(re.findall(r"\b(vmc_[a-z_0-9]+)\s*\(", txt))
+
+
+def test_library_exports_every_declared_symbol():
+    vmc.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", vmc.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (vmc_\w+)", out))
+    declared = _declared()
+    assert declared, "no declarations parsed"
+    assert declared <= exported, declared - exported
+    assert exported <= declared, exported - declared
+
+
+def test_status_strings():
+    lib = vmc.load_library()
+    assert lib.vmc_status_string(0) == b"Success"
+    assert lib.vmc_status_string(1) == b"No lightsource"          # cpp/Errors.hpp:27-28
+    assert b"GaussianSigma" not in lib.vmc_status_string(15)
+    assert lib.vmc_status_string(17).startswith(b"A light source")
+    assert lib.vmc_version().startswith(b"valomc")
+
+
+def _desc(p, keep):
+    def ptr(a, dtype):
+        if a is None:
+            return None
+        arr = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+        keep.append(arr)
+        return arr.ctypes.data
+
+    d = vmc.ProblemDesc()
+    d.dim, d.Ne, d.Nb, d.Nr = p.dim, p.Ne, p.Nb, p.Nr
+    d.H, d.HN, d.BH, d.r = ptr(p.H, np.int64), ptr(p.HN, np.int64), ptr(p.BH, np.int64), ptr(p.r, np.float64)
+    d.mua, d.mus, d.g, d.n = (ptr(a, np.float64) for a in (p.mua, p.mus, p.g, p.n))
+    d.BCType, d.BCn = ptr(p.BCType, np.uint8), ptr(p.BCn, np.float64)
+    d.BCLNormal = ptr(p.BCLNormal, np.float64)
+    d.BCLightDirectionType = ptr(p.BCLDirType, np.uint8)
+    d.BCIntensity = ptr(p.BCIntensity, np.float64)
+    d.GaussianSigma = ptr(p.GaussianSigma, np.float64)
+    return d
+
+
+def _check(p):
+    lib = vmc.load_library()
+    lib.vmc_problem_check.argtypes = [C.POINTER(vmc.ProblemDesc), C.c_void_p, C.POINTER(C.c_int64)]
+    keep = []
+    d = _desc(p, keep)
+    hn = np.zeros(p.Ne * (p.dim + 1), dtype=np.int64)
+    ns = C.c_int64(0)
+    st = lib.vmc_problem_check(C.byref(d), hn.ctypes.data, C.byref(ns))
+    return st, hn.reshape(p.dim + 1, p.Ne).T, ns.value
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_host_prepass_neighbour_table_matches_reference(name):
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    p = cases.from_arrays(d)
+    st, hn, ns = _check(p)
+    assert st == 0
+    ref = d["out_T1_HN"]
+    assert np.array_equal(hn, ref)
+    assert ns > 0
+
+
+@pytest.mark.parametrize("name", sorted(cases.ERROR_CASES))
+def test_host_prepass_error_status_matches_reference(name):
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    st, _, _ = _check(cases.from_arrays(d))
+    assert st == int(d["out_status"])
+
+
+def test_undefined_direction_is_rejected():
+    p = cases.p3_given_L_I()
+    p.BCLDirType = None          # 'L' and 'I' then have no direction (MC3D.hpp:1112-1192)
+    st, _, _ = _check(p)
+    assert st == vmc.STATUS_UNDEFINED_DIRECTION
+
+
+def test_neighbours_on_config_meshes():
+    """Sort-based face matching on the 40 mm benchmark box: every interior
+    face is shared by exactly two tets, every boundary triangle is used
+    once, and the table is symmetric."""
+    from valomc_amd import meshgen as mg
+    p = mg.sphere_box_3d(nphoton=1, side_mm=12)
+    st, hn, ns = _check(p)
+    assert st == 0 and ns == 1
+    inner = hn >= 0
+    el = np.nonzero(inner)
+    nbr = hn[inner]
+    # symmetry: the neighbour lists this element back
+    assert np.all(np.any(hn[nbr] == el[0][:, None], axis=1))
+    bnd = -1 - hn[~inner]
+    assert np.array_equal(np.sort(bnd), np.arange(p.Nb))
+
+
+def test_brute_force_neighbours_small():
+    """Pin the face matcher against an O(Ne^2) vertex-set comparison."""
+    from valomc_amd import meshgen as mg
+    r, H, BH = mg.grid_mesh(np.arange(3) + 0.5, np.arange(2) + 0.5, np.arange(2) + 0.5)
+    p = mg.homogeneous(3, r, H, BH, 0.1, 1, 0.9, 1.4)
+    p.BCType[0] = ord("l")
+    st, hn, _ = _check(p)
+    assert st == 0
+    faces = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    for e in range(p.Ne):
+        for f, fv in enumerate(faces):
+            key = set(H[e, list(fv)])
+            others = [j for j in range(p.Ne) if j != e and key <= set(H[j])]
+            bnds = [-1 - b for b in range(p.Nb) if key == set(BH[b])]
+            cand = others + bnds
+            assert len(cand) == 1
+            assert hn[e, f] == cand[0]

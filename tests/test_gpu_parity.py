@@ -1,0 +1,196 @@
+"""GPU parity: the HIP transport path against the oracle.
+
+* FP64 on the device vs the CPU restatement driven by the same per-photon
+  Philox streams: the same photons take the same paths, so the tallies
+  agree to rounding (relative L2 <= 1e-9) on every golden case.
+* Against the reference's own MT19937 streams the agreement is
+  statistical: relative L2 within a few two-seed spreads of the reference.
+* FP32 transport vs FP64 with the same streams: <= 1% relative L2.
+* Energy ledger, fixed-point reproducibility across launch shapes, abort,
+  error codes and the MC3D/MC2D class mirror.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+OUT = ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")
+
+
+@pytest.fixture(scope="module")
+def vmc():
+    import valomc_amd
+    print("device:", valomc_amd.device_info(), flush=True)
+    return valomc_amd
+
+
+def _golden(name):
+    return cases.from_arrays(dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)))
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a - b)
+
+
+def _run(vmc, p, **kw):
+    with vmc.Problem(p) as prob:
+        return prob.run(kw.pop("nphoton", p.Nphoton), seed=kw.pop("seed", p.seed), f=p.f, phase0=p.phase0, **kw)
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_fp64_matches_oracle_on_same_streams(vmc, name):
+    p = _golden(name)
+    print(f"case {name} Ne={p.Ne} N={p.Nphoton}", flush=True)
+    ref = pyoracle.run(p, 4, pyoracle.RNG_PHILOX)
+    res = _run(vmc, p, precision=vmc.FP64)
+    for k in OUT:
+        a, b = getattr(res, k), getattr(ref, k)
+        if np.linalg.norm(b) == 0:
+            assert np.max(np.abs(a)) < 1e-12, k
+        else:
+            assert rel_l2(a, b) < 1e-9, (k, rel_l2(a, b))
+    assert abs(res.loss - ref.loss) <= max(2, p.Nphoton // 1000), (res.loss, ref.loss)
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_ledger_closes(vmc, name):
+    p = _golden(name)
+    for prec, tol in ((vmc.FP64, 1e-9), (vmc.FP32, 1e-6)):
+        res = _run(vmc, p, precision=prec)
+        absorbed, exited, killed, gained, lost = res.ledger
+        total = absorbed + exited + killed - gained + lost
+        assert abs(total - p.Nphoton) <= tol * p.Nphoton, (prec, total)
+
+
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_cosinic_hetero_n", "p3_direct_abs_fd",
+                                  "p2_square_cosinic", "p2_iso_gauss_fd", "p2_mirror_hetero"])
+def test_fp32_within_one_percent_of_fp64(vmc, name):
+    p = _golden(name)
+    p.Nphoton = 200000
+    a = _run(vmc, p, precision=vmc.FP64)
+    b = _run(vmc, p, precision=vmc.FP32)
+    for k in ("ER", "EBR"):
+        assert rel_l2(getattr(b, k), getattr(a, k)) < 1e-2, (k, rel_l2(getattr(b, k), getattr(a, k)))
+    if p.f > 0:
+        assert rel_l2(b.EI, a.EI) < 1e-2
+
+
+@pytest.mark.parametrize("name,nphot", [("p2_square_cosinic", 400000), ("p3_pencil_sphere", 200000),
+                                        ("p3_iso_mirror", 200000), ("p2_given_directions", 200000)])
+def test_statistical_parity_with_reference_streams(vmc, name, nphot):
+    """GPU (Philox) vs oracle (the reference's MT19937 streams): the
+    difference stays within the spread of two independent reference runs."""
+    p = _golden(name)
+    p.Nphoton = nphot
+    r1 = pyoracle.run(p, 8, pyoracle.RNG_MT19937)
+    p2 = _golden(name)
+    p2.Nphoton, p2.seed = nphot, p.seed + 1000
+    r2 = pyoracle.run(p2, 8, pyoracle.RNG_MT19937)
+    g = _run(vmc, p, precision=vmc.FP64, seed=777)
+    for k in ("ER", "EBR"):
+        spread = rel_l2(getattr(r2, k), getattr(r1, k))
+        diff = rel_l2(getattr(g, k), getattr(r1, k))
+        assert diff < 2.5 * spread, (k, diff, spread)
+    # loss rate agrees with the reference's
+    assert abs(g.loss - r1.loss) <= 5 * np.sqrt(max(r1.loss, 1)) + 5
+
+
+def test_fixed_point_tallies_are_bit_reproducible(vmc):
+    p = _golden("p3_pencil_sphere")
+    p.Nphoton = 100000
+    with vmc.Problem(p) as prob:
+        a = prob.run(p.Nphoton, seed=5, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+        b = prob.run(p.Nphoton, seed=5, precision=vmc.FP32, tally=vmc.TALLY_FIXED64,
+                     block_size=128, replicas=3, refill_min=16, blocks_per_cu=2)
+        c = prob.run(p.Nphoton, seed=5, precision=vmc.FP32, tally=vmc.TALLY_FP64)
+    for k in OUT:
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    assert rel_l2(c.ER, a.ER) < 1e-8
+
+
+def test_results_do_not_depend_on_launch_shape(vmc):
+    p = _golden("p2_square_cosinic")
+    p.Nphoton = 50000
+    with vmc.Problem(p) as prob:
+        a = prob.run(p.Nphoton, seed=9, precision=vmc.FP64)
+        b = prob.run(p.Nphoton, seed=9, precision=vmc.FP64, block_size=64, replicas=1, refill_min=32)
+    for k in OUT:
+        assert rel_l2(getattr(a, k), getattr(b, k)) < 1e-12, k
+
+
+def test_abort_keeps_planned_normalisation(vmc):
+    p = _golden("p2_square_cosinic")
+    p.Nphoton = 200000
+    calls = []
+
+    def progress(pct):
+        calls.append(pct)
+        return len(calls) < 5
+
+    with vmc.Problem(p) as prob:
+        full = prob.run(p.Nphoton, seed=3)
+        part = prob.run(p.Nphoton, seed=3, progress=progress, check=False)
+    assert part.status == vmc.STATUS_ABORTED
+    assert len(calls) == 5
+    # 5 of 20 chunks ran and the tallies are still divided by the planned N
+    # (MC3D.hpp:1714-1718)
+    ratio = part.ER.sum() / full.ER.sum()
+    assert 0.2 < ratio < 0.3
+
+
+@pytest.mark.parametrize("name", sorted(cases.ERROR_CASES))
+def test_error_status_on_device_path(vmc, name):
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    with pytest.raises(vmc.VmcError) as ei:
+        vmc.Problem(cases.from_arrays(d))
+    assert ei.value.status == int(d["out_status"])
+
+
+def test_class_mirror_mc2d(vmc):
+    p = _golden("p2_doc_quad")
+    mc = vmc.MC2D()
+    mc.H, mc.BH, mc.r = p.H, p.BH, p.r
+    mc.mua, mc.mus, mc.g, mc.n = p.mua, p.mus, p.g, p.n
+    mc.BCType, mc.BCn = p.BCType, p.BCn
+    mc.Nphoton, mc.seed = 50000, 4
+    mc.ErrorChecks()
+    mc.Init()
+    assert mc.HN.shape == (2, 3)
+    mc.MonteCarlo()
+    ref = pyoracle.run(p, 1, pyoracle.RNG_MT19937)
+    assert mc.ER.shape == (2,) and np.all(mc.ER > 0)
+    assert rel_l2(mc.ER, ref.ER) < 0.05
+
+
+def test_one_shot_mex_style_entry(vmc):
+    import ctypes as C
+    lib = vmc.load_library()
+    p = _golden("p3_single_cube")
+    cm = lambda a, t: np.ascontiguousarray(np.asarray(a, dtype=t).ravel(order="F"))
+    H, BH, r = cm(p.H, np.int64), cm(p.BH, np.int64), cm(p.r, np.float64)
+    arr = {k: cm(getattr(p, k), np.float64) for k in ("mua", "mus", "g", "n", "BCn")}
+    bct = cm(p.BCType, np.uint8)
+    out = {k: np.zeros(p.Ne if k in ("ER", "EI") else p.Nb) for k in OUT}
+    hn = np.zeros(p.Ne * 4, dtype=np.int64)
+    loss = C.c_int64(0)
+    V = C.c_void_p
+    fn = lib.vmc_monte_carlo
+    fn.argtypes = [C.c_int32, C.c_int64, C.c_int64, C.c_int64, V, V, V, V, V, V, V, V, V, V, V, V, V,
+                   C.c_double, C.c_double, C.c_int64, V, C.c_uint64, C.c_int32, V, V, V, V, V, V, V,
+                   C.POINTER(C.c_int64)]
+    st = fn(3, p.Ne, p.Nb, p.Nr, H.ctypes.data, None, BH.ctypes.data, r.ctypes.data, bct.ctypes.data, None,
+            None, None, arr["BCn"].ctypes.data, arr["mua"].ctypes.data, arr["mus"].ctypes.data,
+            arr["g"].ctypes.data, arr["n"].ctypes.data, 0.0, 0.0, 20000, None, 11, 0,
+            *[out[k].ctypes.data for k in OUT], hn.ctypes.data, C.byref(loss))
+    assert st == 0
+    p.Nphoton, p.seed = 20000, 11
+    ref = pyoracle.run(p, 2, pyoracle.RNG_PHILOX)
+    assert rel_l2(out["ER"], ref.ER) < 1e-9
+    assert np.array_equal(hn.reshape(4, p.Ne).T, ref.HN)

@@ -1,0 +1,315 @@
+"""MI355X photon-transport engine with ValoMC's MC3D/MC2D interface.
+
+Python binding of the C ABI in include/valomc.h (libvalomc.so, built
+in-tree by ``python -m valomc_amd.build``).  ``MC3D``/``MC2D`` mirror the
+reference engine classes (cpp/3d/MC3D.hpp:51-173, cpp/2d/MC2D.hpp:59-180):
+set the public arrays, call ``ErrorChecks()``, ``Init()`` and
+``MonteCarlo()``, read ``ER, EI, EBR, EBI, DEBR, DEBI`` and ``loss``.
+There is no CPU fallback: without the HIP library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvalomc.so")
+
+FP64 = 0
+FP32 = 1
+TALLY_FP64 = 0
+TALLY_FIXED64 = 1
+
+# reference mcerror order (cpp/Errors.hpp:4-21); status = 1 + code
+MCERRORS = (
+    "NO_LIGHTSOURCE", "SIZE_MISMATCH_G", "SIZE_MISMATCH_MUA", "SIZE_MISMATCH_MUS",
+    "SIZE_MISMATCH_N", "SIZE_MISMATCH_BCN", "MISSING_BCN", "SIZE_MISMATCH_BCTYPE",
+    "SIZE_MISMATCH_LIGHT_DIRECTION", "SIZE_MISMATCH_LIGHT_DIRECTION_TYPE", "INCONSISTENT_H",
+    "INCONSISTENT_BH", "INCONSISTENT_MESH", "INCONSISTENT_MESH_DUPLICATE_NEIGHBORS",
+    "NO_GAUSSIAN_SIGMA", "MISSING_BOUNDARY",
+)
+STATUS_UNDEFINED_DIRECTION = 17
+STATUS_INVALID_ARGUMENT = 18
+STATUS_ABORTED = 103
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("Ne", C.c_int64), ("Nb", C.c_int64), ("Nr", C.c_int64),
+        ("H", C.c_void_p), ("HN", C.c_void_p), ("BH", C.c_void_p), ("r", C.c_void_p),
+        ("mua", C.c_void_p), ("mus", C.c_void_p), ("g", C.c_void_p), ("n", C.c_void_p),
+        ("BCType", C.c_void_p), ("BCn", C.c_void_p), ("BCLNormal", C.c_void_p),
+        ("BCLightDirectionType", C.c_void_p), ("BCIntensity", C.c_void_p),
+        ("GaussianSigma", C.c_void_p),
+    ]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("f", C.c_double), ("phase0", C.c_double), ("Nphoton", C.c_int64), ("seed", C.c_uint64),
+        ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
+        ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
+        ("stream_tag", C.c_uint32), ("reserved", C.c_int32 * 5),
+    ]
+
+
+class Output(C.Structure):
+    _fields_ = [("ER", C.c_void_p), ("EI", C.c_void_p), ("EBR", C.c_void_p), ("EBI", C.c_void_p),
+                ("DEBR", C.c_void_p), ("DEBI", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("photons", C.c_int64), ("loss", C.c_int64), ("steps", C.c_int64),
+        ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("ledger", C.c_double * 5),
+        ("rechecks", C.c_int64), ("grid_blocks", C.c_int32), ("block_size", C.c_int32),
+    ]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_void_p)
+
+
+class VmcError(RuntimeError):
+    """Non-zero status from the C ABI."""
+
+    def __init__(self, status: int, lib=None):
+        msg = _status_string(status)
+        if lib is not None and status >= 100:
+            detail = lib.vmc_last_error().decode()
+            if detail:
+                msg += f" ({detail})"
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+        self.mcerror = MCERRORS[status - 1] if 1 <= status <= 16 else None
+
+
+_lib = None
+
+
+def load_library():
+    """Load libvalomc.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built: run `python -m valomc_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        lib.vmc_problem_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
+        lib.vmc_problem_run.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.POINTER(Output),
+                                        C.POINTER(Stats), C.c_void_p, C.c_void_p]
+        lib.vmc_problem_get_hn.argtypes = [C.c_void_p, C.c_void_p]
+        lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
+        lib.vmc_problem_destroy.argtypes = [C.c_void_p]
+        lib.vmc_problem_destroy.restype = None
+        lib.vmc_status_string.restype = C.c_char_p
+        lib.vmc_last_error.restype = C.c_char_p
+        lib.vmc_version.restype = C.c_char_p
+        lib.vmc_device_info.argtypes = [C.c_char_p, C.c_int32, C.POINTER(C.c_int32)]
+        _lib = lib
+    return _lib
+
+
+def _status_string(st: int) -> str:
+    try:
+        return load_library().vmc_status_string(st).decode()
+    except FileNotFoundError:
+        return "unknown"
+
+
+def device_info() -> str:
+    lib = load_library()
+    buf = C.create_string_buffer(256)
+    ncu = C.c_int32(0)
+    st = lib.vmc_device_info(buf, 256, C.byref(ncu))
+    if st:
+        raise VmcError(st, lib)
+    return buf.value.decode()
+
+
+def _colmajor(a, dtype):
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+
+
+@dataclass
+class RunResult:
+    ER: np.ndarray
+    EI: np.ndarray
+    EBR: np.ndarray
+    EBI: np.ndarray
+    DEBR: np.ndarray
+    DEBI: np.ndarray
+    loss: int
+    steps: int
+    kernel_ms: float
+    total_ms: float
+    ledger: np.ndarray
+    photons: int
+    grid_blocks: int
+    block_size: int
+    status: int = 0
+
+
+class Problem:
+    """A mesh + medium + boundary set resident on the device
+    (vmc_problem_create ... vmc_problem_destroy)."""
+
+    def __init__(self, p):
+        """`p` has the fields of ``meshgen.Problem`` (row-per-element arrays)."""
+        self.lib = load_library()
+        self.dim, self.Ne, self.Nb, self.Nr = int(p.dim), int(p.Ne), int(p.Nb), int(p.Nr)
+        keep = []
+
+        def ptr(a, dtype):
+            if a is None:
+                return None
+            arr = _colmajor(a, dtype)
+            keep.append(arr)
+            return arr.ctypes.data
+
+        d = ProblemDesc()
+        d.dim, d.Ne, d.Nb, d.Nr = self.dim, self.Ne, self.Nb, self.Nr
+        d.H, d.HN, d.BH, d.r = ptr(p.H, np.int64), ptr(p.HN, np.int64), ptr(p.BH, np.int64), ptr(p.r, np.float64)
+        d.mua, d.mus = ptr(p.mua, np.float64), ptr(p.mus, np.float64)
+        d.g, d.n = ptr(p.g, np.float64), ptr(p.n, np.float64)
+        d.BCType, d.BCn = ptr(p.BCType, np.uint8), ptr(p.BCn, np.float64)
+        d.BCLNormal = ptr(getattr(p, "BCLNormal", None), np.float64)
+        d.BCLightDirectionType = ptr(getattr(p, "BCLDirType", None), np.uint8)
+        d.BCIntensity = ptr(getattr(p, "BCIntensity", None), np.float64)
+        d.GaussianSigma = ptr(getattr(p, "GaussianSigma", None), np.float64)
+        h = C.c_void_p()
+        st = self.lib.vmc_problem_create(C.byref(d), C.byref(h))
+        if st:
+            raise VmcError(st, self.lib)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.vmc_problem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def hn(self) -> np.ndarray:
+        nv = self.dim + 1
+        out = np.zeros(self.Ne * nv, dtype=np.int64)
+        st = self.lib.vmc_problem_get_hn(self._h, out.ctypes.data)
+        if st:
+            raise VmcError(st, self.lib)
+        return out.reshape(nv, self.Ne).T.copy()
+
+    def set_medium(self, mua=None, mus=None, g=None, n=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mua, mus, g, n)]
+        st = self.lib.vmc_problem_set_medium(self._h, *[None if a is None else a.ctypes.data for a in arrs])
+        if st:
+            raise VmcError(st, self.lib)
+
+    def run(self, nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
+            precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
+            blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
+            progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
+        rd = RunDesc()
+        rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
+        rd.precision, rd.tally = int(precision), int(tally)
+        rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
+        rd.refill_min, rd.stream_tag = int(refill_min), int(stream_tag)
+        out = dict(ER=np.zeros(self.Ne), EI=np.zeros(self.Ne), EBR=np.zeros(self.Nb),
+                   EBI=np.zeros(self.Nb), DEBR=np.zeros(self.Nb), DEBI=np.zeros(self.Nb))
+        o = Output(*[out[k].ctypes.data for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")])
+        s = Stats()
+        cb = None
+        if progress is not None:
+            cb = PROGRESS_FN(lambda pct, user: 1 if progress(pct) else 0)
+        st = self.lib.vmc_problem_run(self._h, C.byref(rd), C.byref(o), C.byref(s),
+                                      C.cast(cb, C.c_void_p) if cb else None, None)
+        if st and (check and st != STATUS_ABORTED):
+            raise VmcError(st, self.lib)
+        return RunResult(status=st, loss=int(s.loss), steps=int(s.steps), kernel_ms=float(s.kernel_ms),
+                         total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)),
+                         photons=int(s.photons), grid_blocks=int(s.grid_blocks),
+                         block_size=int(s.block_size), **out)
+
+
+class _EngineBase:
+    """Public-member mirror of MC3D/MC2D (MC3D.hpp:107-172)."""
+
+    dim = 3
+
+    def __init__(self):
+        self.H = self.HN = self.BH = self.r = None
+        self.mua = self.mus = self.g = self.n = None
+        self.BCType = self.BCLNormal = self.BCn = self.BCIntensity = None
+        self.BCLightDirectionType = None
+        self.GaussianSigma = None
+        self.f = 0.0
+        self.phase0 = 0.0
+        self.Nphoton = 1
+        self.seed = 5489
+        self.loss = 0
+        self.weight0, self.chance, self.c0 = 0.001, 0.1, 2.99792458e11
+        self.precision = FP64
+        self.tally = TALLY_FP64
+        self.ER = self.EI = self.EBR = self.EBI = self.DEBR = self.DEBI = None
+        self._problem: Optional[Problem] = None
+        self.stats: Optional[RunResult] = None
+
+    def _as_problem(self):
+        from types import SimpleNamespace
+        H = np.asarray(self.H)
+        BCT = self.BCType
+        if isinstance(BCT, (str, bytes)):
+            BCT = np.frombuffer(BCT.encode() if isinstance(BCT, str) else BCT, dtype=np.uint8)
+        ldt = self.BCLightDirectionType
+        if isinstance(ldt, (str, bytes)):
+            ldt = np.frombuffer(ldt.encode() if isinstance(ldt, str) else ldt, dtype=np.uint8)
+        return SimpleNamespace(dim=self.dim, Ne=H.shape[0], Nb=np.asarray(self.BH).shape[0],
+                               Nr=np.asarray(self.r).shape[0], H=H, HN=self.HN, BH=self.BH, r=self.r,
+                               mua=self.mua, mus=self.mus, g=self.g, n=self.n, BCType=BCT,
+                               BCn=self.BCn, BCLNormal=self.BCLNormal, BCLDirType=ldt,
+                               BCIntensity=self.BCIntensity, GaussianSigma=self.GaussianSigma)
+
+    def ErrorChecks(self):
+        """Validation happens with Init() on the device side; kept for API parity."""
+        return None
+
+    def Init(self):
+        if self._problem is not None:
+            self._problem.close()
+        self._problem = Problem(self._as_problem())
+        self.HN = self._problem.hn()
+
+    def MonteCarlo(self, progress: Optional[Callable[[float], bool]] = None, finalchecks=None):
+        if self._problem is None:
+            self.Init()
+        res = self._problem.run(self.Nphoton, seed=self.seed, f=self.f, phase0=self.phase0,
+                                precision=self.precision, tally=self.tally, progress=progress,
+                                check=False)
+        if res.status not in (0, STATUS_ABORTED):
+            raise VmcError(res.status, self._problem.lib)
+        for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI"):
+            setattr(self, k, getattr(res, k))
+        self.loss = res.loss
+        self.stats = res
+        if finalchecks is not None:
+            finalchecks(int(self.Nphoton) if res.status == 0 else -1, int(self.Nphoton))
+        return res
+
+
+class MC3D(_EngineBase):
+    dim = 3
+
+
+class MC2D(_EngineBase):
+    dim = 2

@@ -1,0 +1,974 @@
+// Host side of the C ABI (include/valomc.h).
+//
+// vmc_problem_create is the reference's ErrorChecks() + Init()
+// (cpp/3d/MC3D.hpp:460-738, cpp/2d/MC2D.hpp:390-643): same checks in the same
+// order with the same error codes, the same neighbour table, the same light
+// source list and CDF, the same BCLNormal preprocessing.  It then packs the
+// element records the kernels walk on and keeps them resident on the device.
+// vmc_problem_run is MonteCarlo() (MC3D.hpp:1641-1822): it launches the
+// persistent transport kernel, sums the tally replicas on the device and
+// normalises on the host with the reference's own expressions.
+#include "../../include/valomc.h"
+#include "vmc_types.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace vmc {
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, int *blocks_per_cu);
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, hipStream_t s,
+                       const WalkParams &P);
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
+hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, double inv_scale,
+                         hipStream_t s);
+} // namespace vmc
+
+using namespace vmc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+// mcerror codes (cpp/Errors.hpp:4-21); the ABI reports 1 + code
+enum McErr {
+  NO_LIGHTSOURCE, SIZE_MISMATCH_G, SIZE_MISMATCH_MUA, SIZE_MISMATCH_MUS, SIZE_MISMATCH_N,
+  SIZE_MISMATCH_BCN, MISSING_BCN, SIZE_MISMATCH_BCTYPE, SIZE_MISMATCH_LIGHT_DIRECTION,
+  SIZE_MISMATCH_LIGHT_DIRECTION_TYPE, INCONSISTENT_H, INCONSISTENT_BH, INCONSISTENT_MESH,
+  INCONSISTENT_MESH_DUPLICATE_NEIGHBORS, NO_GAUSSIAN_SIGMA, MISSING_BOUNDARY
+};
+inline int mc_status(int e) { return 1 + e; }
+
+const int kFace3[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}}; // MC3D.hpp:1013
+const int kEdge2[3][2] = {{0, 1}, {1, 2}, {2, 0}};                     // MC2D.hpp:921
+const int64_t kUnset = INT64_MAX;                                      // MC3D.hpp:29-31
+const double kC0 = 2.99792458e11;                                      // MC3D.hpp:178
+
+struct HipFail {
+  hipError_t e;
+  const char *what;
+};
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw HipFail{e, what};
+}
+
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    hip_check(hipMalloc((void **)&p, count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+} // namespace
+
+struct vmc_problem {
+  int dim = 3, nv = 4, nbv = 3;
+  int64_t Ne = 0, Nb = 0, Nr = 0;
+  std::vector<int64_t> H, HN, BH;
+  std::vector<double> r, mua, mus, g, n;
+  std::vector<char> BCType, BCLDirType;
+  std::vector<double> BCn, BCLNormal, BCIntensity, GaussianSigma;
+  double shift[3] = {0, 0, 0};
+  std::vector<double> vol, area;    // element volumes (2D areas), boundary areas (2D lengths)
+  std::vector<int64_t> lsFace, lsMother;
+  std::vector<double> lsCDF;
+  std::vector<SrcRec> src;
+  std::vector<BndRec> bnd;
+  std::vector<i4> nbcode;
+
+  int device = 0, num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf<unsigned char> elems32, elems64;
+  DevBuf<double> cdf;
+  DevBuf<SrcRec> dsrc;
+  DevBuf<BndRec> dbnd;
+  DevBuf<unsigned long long> etally, btally, counters;
+  DevBuf<double> ledger, raw_e, raw_b;
+  bool params_dirty32 = true, params_dirty64 = true;
+
+  int64_t h(int64_t el, int c) const { return H[el + Ne * c]; }
+  int64_t hn(int64_t el, int c) const { return HN[el + Ne * c]; }
+  int64_t bh(int64_t ib, int c) const { return BH[ib + Nb * c]; }
+  double x(int64_t node, int d) const { return r[node + Nr * d]; }
+  double bcl(int64_t ib, int d) const { return BCLNormal[ib + Nb * d]; }
+  double &bcl_ref(int64_t ib, int d) { return BCLNormal[ib + Nb * d]; }
+};
+
+namespace {
+
+struct McError {
+  int code;
+};
+
+// ---------------------------------------------------------------------------
+// Geometry with the reference's expressions (MC3D.hpp:306-457, MC2D.hpp:303-387)
+// ---------------------------------------------------------------------------
+
+double volume3(const vmc_problem &m, int64_t el) {
+  double ax = m.x(m.h(el, 0), 0), ay = m.x(m.h(el, 0), 1), az = m.x(m.h(el, 0), 2);
+  double bx = m.x(m.h(el, 1), 0), by = m.x(m.h(el, 1), 1), bz = m.x(m.h(el, 1), 2);
+  double cx = m.x(m.h(el, 2), 0), cy = m.x(m.h(el, 2), 1), cz = m.x(m.h(el, 2), 2);
+  double dx = m.x(m.h(el, 3), 0), dy = m.x(m.h(el, 3), 1), dz = m.x(m.h(el, 3), 2);
+  double v = (ax - dx) * ((by - dy) * (cz - dz) - (bz - dz) * (cy - dy)) -
+             (ay - dy) * ((bx - dx) * (cz - dz) - (bz - dz) * (cx - dx)) +
+             (az - dz) * ((bx - dx) * (cy - dy) - (by - dy) * (cx - dx));
+  return fabs(v) / 6.0;
+}
+
+double heron3(const vmc_problem &m, int64_t i0, int64_t i1, int64_t i2) {
+  double a = sqrt(pow(m.x(i1, 0) - m.x(i0, 0), 2) + pow(m.x(i1, 1) - m.x(i0, 1), 2) + pow(m.x(i1, 2) - m.x(i0, 2), 2));
+  double b = sqrt(pow(m.x(i2, 0) - m.x(i0, 0), 2) + pow(m.x(i2, 1) - m.x(i0, 1), 2) + pow(m.x(i2, 2) - m.x(i0, 2), 2));
+  double c = sqrt(pow(m.x(i1, 0) - m.x(i2, 0), 2) + pow(m.x(i1, 1) - m.x(i2, 1), 2) + pow(m.x(i1, 2) - m.x(i2, 2), 2));
+  return sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c)) / 4.0;
+}
+
+double area2(const vmc_problem &m, int64_t el) {
+  double ax = m.x(m.h(el, 1), 0) - m.x(m.h(el, 0), 0);
+  double ay = m.x(m.h(el, 1), 1) - m.x(m.h(el, 0), 1);
+  double bx = m.x(m.h(el, 2), 0) - m.x(m.h(el, 0), 0);
+  double by = m.x(m.h(el, 2), 1) - m.x(m.h(el, 0), 1);
+  double c = ax * by - ay * bx;
+  return 0.5 * (c >= 0.0 ? c : -c);
+}
+
+double len2(const vmc_problem &m, int64_t i0, int64_t i1) {
+  return sqrt(pow(m.x(i1, 0) - m.x(i0, 0), 2) + pow(m.x(i1, 1) - m.x(i0, 1), 2));
+}
+
+void bnd_normal(const vmc_problem &m, int64_t ib, double *nrm) {
+  if (m.dim == 3) {
+    int64_t i0 = m.bh(ib, 0), i1 = m.bh(ib, 1), i2 = m.bh(ib, 2);
+    double x1 = m.x(i1, 0) - m.x(i0, 0), y1 = m.x(i1, 1) - m.x(i0, 1), z1 = m.x(i1, 2) - m.x(i0, 2);
+    double x2 = m.x(i2, 0) - m.x(i0, 0), y2 = m.x(i2, 1) - m.x(i0, 1), z2 = m.x(i2, 2) - m.x(i0, 2);
+    double nx = y1 * z2 - z1 * y2, ny = z1 * x2 - x1 * z2, nz = x1 * y2 - y1 * x2;
+    double len = sqrt(nx * nx + ny * ny + nz * nz);
+    nrm[0] = nx / len; nrm[1] = ny / len; nrm[2] = nz / len;
+  } else {
+    int64_t i0 = m.bh(ib, 0), i1 = m.bh(ib, 1);
+    double dx = m.x(i1, 0) - m.x(i0, 0), dy = m.x(i1, 1) - m.x(i0, 1);
+    double len = sqrt(dx * dx + dy * dy);
+    nrm[0] = dy / len; nrm[1] = -dx / len; nrm[2] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ErrorChecks (MC3D.hpp:460-571, MC2D.hpp:390-513)
+// ---------------------------------------------------------------------------
+
+void error_checks(const vmc_problem &m) {
+  if (m.BCn.empty()) throw McError{MISSING_BCN};
+  bool has_ls = false, has_gauss = false;
+  for (char c : m.BCType) {
+    if (c != 'a') has_ls = true;
+    if (c == 'g') has_gauss = true;
+  }
+  if (!has_ls) throw McError{NO_LIGHTSOURCE};
+  if (m.dim == 2 && has_gauss && m.GaussianSigma.empty()) throw McError{NO_GAUSSIAN_SIGMA};
+  for (int64_t v : m.H)
+    if (v < 0 || v >= m.Nr) throw McError{INCONSISTENT_H};
+  for (int64_t v : m.BH)
+    if (v < 0 || v >= m.Nr) throw McError{INCONSISTENT_BH};
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour table.  Same result as the reference's vertex-incidence search
+// (MC3D.hpp:740-871, MC2D.hpp:645-773): face f of element e gets the other
+// element, or boundary entry -1-ib, holding all of its vertices; two
+// different matches are an inconsistent mesh.  Found here by sorting face
+// keys, O(F log F) instead of O(sum of vertex degree^2).
+// ---------------------------------------------------------------------------
+
+struct FaceKey {
+  int64_t a, b, c;
+  int64_t owner;   // (el << 2 | f) for element faces, -1 - ib for boundary entries
+  bool operator<(const FaceKey &o) const {
+    if (a != o.a) return a < o.a;
+    if (b != o.b) return b < o.b;
+    if (c != o.c) return c < o.c;
+    return owner < o.owner;
+  }
+  bool same(const FaceKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+
+inline void sort3(int64_t &a, int64_t &b, int64_t &c) {
+  if (a > b) std::swap(a, b);
+  if (b > c) std::swap(b, c);
+  if (a > b) std::swap(a, b);
+}
+
+void build_neighbours(vmc_problem &m) {
+  if ((int64_t)m.HN.size() == m.Ne * m.nv) return;   // given (MC3D.hpp:839)
+  m.HN.assign(m.Ne * m.nv, kUnset);
+  std::vector<FaceKey> keys;
+  keys.reserve(m.Ne * m.nv + m.Nb);
+  for (int64_t e = 0; e < m.Ne; ++e)
+    for (int f = 0; f < m.nv; ++f) {
+      FaceKey k;
+      if (m.dim == 3) {
+        k.a = m.h(e, kFace3[f][0]); k.b = m.h(e, kFace3[f][1]); k.c = m.h(e, kFace3[f][2]);
+        sort3(k.a, k.b, k.c);
+      } else {
+        k.a = std::min(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        k.b = std::max(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        k.c = -1;
+      }
+      k.owner = (e << 2) | f;
+      keys.push_back(k);
+    }
+  for (int64_t b = 0; b < m.Nb; ++b) {
+    FaceKey k;
+    if (m.dim == 3) {
+      k.a = m.bh(b, 0); k.b = m.bh(b, 1); k.c = m.bh(b, 2);
+      sort3(k.a, k.b, k.c);
+    } else {
+      k.a = std::min(m.bh(b, 0), m.bh(b, 1));
+      k.b = std::max(m.bh(b, 0), m.bh(b, 1));
+      k.c = -1;
+    }
+    k.owner = -1 - b;
+    keys.push_back(k);
+  }
+  std::sort(keys.begin(), keys.end());
+  const int dup = m.dim == 3 ? INCONSISTENT_MESH_DUPLICATE_NEIGHBORS : INCONSISTENT_MESH;
+  size_t i = 0;
+  while (i < keys.size()) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].same(keys[i])) ++j;
+    // every element face in the group takes the one other entry
+    for (size_t q = i; q < j; ++q) {
+      if (keys[q].owner < 0) continue;
+      const int64_t e = keys[q].owner >> 2;
+      const int f = (int)(keys[q].owner & 3);
+      int64_t val = kUnset;
+      for (size_t o = i; o < j; ++o) {
+        if (o == q) continue;
+        const int64_t cand = keys[o].owner < 0 ? keys[o].owner : (keys[o].owner >> 2);
+        if (cand == e) continue;
+        if (val != kUnset && val != cand) throw McError{dup};
+        val = cand;
+      }
+      m.HN[e + m.Ne * f] = val;
+    }
+    i = j;
+  }
+  if (m.dim == 2)
+    for (int64_t v : m.HN)
+      if (v == kUnset) throw McError{MISSING_BOUNDARY};   // MC2D.hpp:768-771
+}
+
+bool is_source(int dim, char t) {
+  if (dim == 3) return t == 'l' || t == 'L' || t == 'i' || t == 'I' || t == 'c' || t == 'C' || t == 'p';
+  return t == 'l' || t == 'i' || t == 'c' || t == 'g';
+}
+
+// BuildLightSource (MC3D.hpp:941-1002, MC2D.hpp:851-911): element-major order
+void build_light_sources(vmc_problem &m) {
+  m.lsFace.clear(); m.lsMother.clear(); m.lsCDF.clear();
+  for (int64_t e = 0; e < m.Ne; ++e)
+    for (int f = 0; f < m.nv; ++f) {
+      int64_t v = m.hn(e, f);
+      if (v <= -1) {
+        int64_t ib = -1 - v;
+        if (ib < m.Nb && is_source(m.dim, m.BCType[ib])) {
+          m.lsFace.push_back(ib);
+          m.lsMother.push_back(e);
+          m.lsCDF.push_back(m.dim == 3 ? heron3(m, m.bh(ib, 0), m.bh(ib, 1), m.bh(ib, 2))
+                                       : len2(m, m.bh(ib, 0), m.bh(ib, 1)));
+        }
+      }
+    }
+  const size_t ns = m.lsCDF.size();
+  if (ns == 0) throw McError{NO_LIGHTSOURCE};
+  for (size_t i = 0; i < ns; ++i) m.lsCDF[i] *= m.BCIntensity[m.lsFace[i]];
+  for (size_t i = 1; i < ns; ++i) m.lsCDF[i] += m.lsCDF[i - 1];
+  for (size_t i = 0; i < ns; ++i) m.lsCDF[i] /= m.lsCDF[ns - 1];
+}
+
+// BCLNormal preprocessing of Init (MC3D.hpp:670-735, MC2D.hpp:611-640)
+void prepare_directions(vmc_problem &m) {
+  if (m.BCLNormal.empty()) return;
+  if (m.dim == 3) {
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      double len = sqrt(pow(m.bcl(i, 0), 2) + pow(m.bcl(i, 1), 2) + pow(m.bcl(i, 2), 2));
+      if (m.BCLDirType[i] != 'n') continue;   // only 'n' rows are normalised
+      for (int d = 0; d < 3; ++d) m.bcl_ref(i, d) /= len;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      if (m.BCLDirType[i] != 'r') continue;
+      double nn[3], e1[3], e2[3], len, dpn, dp1;
+      bnd_normal(m, i, nn);
+      for (int d = 0; d < 3; ++d) {
+        e1[d] = m.x(m.bh(i, 1), d) - m.x(m.bh(i, 0), d);
+        e2[d] = m.x(m.bh(i, 2), d) - m.x(m.bh(i, 0), d);
+      }
+      dpn = nn[0] * e1[0] + nn[1] * e1[1] + nn[2] * e1[2];
+      for (int d = 0; d < 3; ++d) e1[d] -= dpn * nn[d];
+      len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+      for (int d = 0; d < 3; ++d) e1[d] /= len;
+      dpn = nn[0] * e2[0] + nn[1] * e2[1] + nn[2] * e2[2];
+      dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+      for (int d = 0; d < 3; ++d) e2[d] -= dpn * nn[d] + dp1 * e1[d];
+      len = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+      for (int d = 0; d < 3; ++d) e2[d] /= len;
+      double ox = m.bcl(i, 0), oy = m.bcl(i, 1), oz = m.bcl(i, 2);
+      for (int d = 0; d < 3; ++d) m.bcl_ref(i, d) = ox * e1[d] + oy * e2[d] + oz * nn[d];
+    }
+  } else {
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      double len = sqrt(pow(m.bcl(i, 0), 2) + pow(m.bcl(i, 1), 2));
+      m.bcl_ref(i, 0) /= len;
+      m.bcl_ref(i, 1) /= len;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      if (m.BCLDirType[i] != 'r') continue;
+      double nn[3], t[2], len;
+      bnd_normal(m, i, nn);
+      t[0] = m.x(m.bh(i, 1), 0) - m.x(m.bh(i, 0), 0);
+      t[1] = m.x(m.bh(i, 1), 1) - m.x(m.bh(i, 0), 1);
+      len = sqrt(t[0] * t[0] + t[1] * t[1]);
+      t[0] /= len; t[1] /= len;
+      double ox = m.bcl(i, 0), oy = m.bcl(i, 1);
+      m.bcl_ref(i, 0) = ox * t[0] + oy * nn[0];
+      m.bcl_ref(i, 1) = ox * t[1] + oy * nn[1];
+    }
+  }
+}
+
+// Per-source launch records (CreatePhoton, MC3D.hpp:1060-1294 / MC2D.hpp:958-1132)
+void build_source_records(vmc_problem &m) {
+  m.src.assign(m.lsFace.size(), SrcRec{});
+  for (size_t i = 0; i < m.lsFace.size(); ++i) {
+    SrcRec &s = m.src[i];
+    const int64_t ib = m.lsFace[i], el = m.lsMother[i];
+    s.ib = (int32_t)ib;
+    s.mother = (int32_t)el;
+    s.face = -1;
+    for (int f = 0; f < m.nv; ++f)
+      if (-1 - m.hn(el, f) == ib) { s.face = f; break; }
+    const int nb = m.nbv;
+    double *vv[3] = {s.v0, s.v1, s.v2};
+    for (int c = 0; c < nb; ++c)
+      for (int d = 0; d < m.dim; ++d) vv[c][d] = m.x(m.bh(ib, c), d) - m.shift[d];
+    // inward normal: orient towards the element centroid (MC3D.hpp:1092-1107)
+    double nn[3];
+    bnd_normal(m, ib, nn);
+    double t[3] = {0, 0, 0};
+    for (int d = 0; d < m.dim; ++d) {
+      double cen = 0, fc = 0;
+      for (int c = 0; c < m.nv; ++c) cen += m.x(m.h(el, c), d);
+      for (int c = 0; c < nb; ++c) fc += m.x(m.bh(ib, c), d);
+      t[d] = cen / m.nv - fc / nb;
+    }
+    double dot = 0;
+    for (int d = 0; d < m.dim; ++d) dot += nn[d] * t[d];
+    if (dot < 0.0)
+      for (int d = 0; d < 3; ++d) nn[d] = -nn[d];
+    for (int d = 0; d < 3; ++d) s.nin[d] = nn[d];
+    const char bt = m.BCType[ib];
+    const bool given = m.BCLDirType[ib] != 'n';
+    if (m.dim == 3) {
+      if (!given && (bt == 'L' || bt == 'I' || bt == 'p')) throw McError{-1 + VMC_ERR_UNDEFINED_DIRECTION};
+      if (bt == 'l' || bt == 'L') {
+        s.kind = given ? kSrcGiven : kSrcNormal;
+        if (given)
+          for (int d = 0; d < 3; ++d) s.dir[d] = m.bcl(ib, d);
+      } else if (bt == 'p') {
+        s.kind = kSrcPencil;
+        for (int d = 0; d < 3; ++d) s.dir[d] = m.bcl(ib, d) - m.shift[d];
+      } else if (bt == 'i' || bt == 'I') {
+        s.kind = kSrcIso3;
+      } else {   // 'c', 'C'
+        s.kind = kSrcCos3;
+        double e1[3], e2[3], len, dpn, dp1;
+        for (int d = 0; d < 3; ++d) {
+          e1[d] = m.x(m.bh(ib, 1), d) - m.x(m.bh(ib, 0), d);
+          e2[d] = m.x(m.bh(ib, 2), d) - m.x(m.bh(ib, 0), d);
+        }
+        if (!given) {
+          len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+          for (int d = 0; d < 3; ++d) e1[d] /= len;
+          dpn = nn[0] * e2[0] + nn[1] * e2[1] + nn[2] * e2[2];
+          dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+          for (int d = 0; d < 3; ++d) e2[d] -= dpn * nn[d] + dp1 * e1[d];
+        } else {
+          // frame orthogonal to BCLNormal, first axis still the inward
+          // normal (MC3D.hpp:1259-1285)
+          double b[3] = {m.bcl(ib, 0), m.bcl(ib, 1), m.bcl(ib, 2)};
+          dpn = b[0] * e1[0] + b[1] * e1[1] + b[2] * e1[2];
+          for (int d = 0; d < 3; ++d) e1[d] -= dpn * b[d];
+          len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+          for (int d = 0; d < 3; ++d) e1[d] /= len;
+          dpn = b[0] * e2[0] + b[1] * e2[1] + b[2] * e2[2];
+          dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+          for (int d = 0; d < 3; ++d) e2[d] -= dpn * b[d] + dp1 * e1[d];
+        }
+        len = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        for (int d = 0; d < 3; ++d) {
+          e2[d] /= len;
+          s.a1[d] = e1[d];
+          s.a2[d] = e2[d];
+        }
+      }
+    } else {
+      s.phi0 = given ? atan2(m.bcl(ib, 1), m.bcl(ib, 0)) : atan2(nn[1], nn[0]);
+      if (bt == 'l') {
+        s.kind = given ? kSrcGiven : kSrcNormal;
+        if (given) { s.dir[0] = m.bcl(ib, 0); s.dir[1] = m.bcl(ib, 1); }
+      } else if (bt == 'i') {
+        s.kind = kSrcIso2;
+      } else if (bt == 'c') {
+        s.kind = given ? kSrcCos2Rej : kSrcCos2;
+      } else {   // 'g'
+        s.kind = given ? kSrcGauss2Rej : kSrcGauss2;
+        s.gsig = m.GaussianSigma[ib];
+        s.gh = 0.5 - 0.5 * erf(M_PI / s.gsig / 2.0 / sqrt(2.0));
+      }
+    }
+  }
+}
+
+// Element records: outward face planes, neighbour codes, optical parameters.
+template <int NF, class R>
+void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
+  out.assign(m.Ne, Elem<NF, R>{});
+  for (int64_t e = 0; e < m.Ne; ++e) {
+    double v[4][3] = {{0}};
+    for (int c = 0; c < m.nv; ++c)
+      for (int d = 0; d < m.dim; ++d) v[c][d] = m.x(m.h(e, c), d) - m.shift[d];
+    Elem<NF, R> &E = out[e];
+    for (int f = 0; f < NF; ++f) {
+      double nrm[3] = {0, 0, 0}, off = 0;
+      if (m.dim == 3) {
+        const double *a = v[kFace3[f][0]], *b = v[kFace3[f][1]], *c = v[kFace3[f][2]];
+        const double *o = v[6 - kFace3[f][0] - kFace3[f][1] - kFace3[f][2]];
+        double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        double w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        nrm[0] = u[1] * w[2] - u[2] * w[1];
+        nrm[1] = u[2] * w[0] - u[0] * w[2];
+        nrm[2] = u[0] * w[1] - u[1] * w[0];
+        double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int d = 0; d < 3; ++d) nrm[d] /= len;
+        if (nrm[0] * (o[0] - a[0]) + nrm[1] * (o[1] - a[1]) + nrm[2] * (o[2] - a[2]) > 0)
+          for (int d = 0; d < 3; ++d) nrm[d] = -nrm[d];
+        for (int d = 0; d < 3; ++d) off += nrm[d] * (a[d] + b[d] + c[d]) / 3.0;
+      } else {
+        const double *a = v[kEdge2[f][0]], *b = v[kEdge2[f][1]];
+        const double *o = v[3 - kEdge2[f][0] - kEdge2[f][1]];
+        double dx = b[0] - a[0], dy = b[1] - a[1];
+        double len = sqrt(dx * dx + dy * dy);
+        nrm[0] = dy / len;
+        nrm[1] = -dx / len;
+        if (nrm[0] * (o[0] - a[0]) + nrm[1] * (o[1] - a[1]) > 0) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; }
+        off = nrm[0] * (a[0] + b[0]) / 2.0 + nrm[1] * (a[1] + b[1]) / 2.0;
+      }
+      E.pl[f].x = (R)nrm[0];
+      E.pl[f].y = (R)nrm[1];
+      E.pl[f].z = (R)nrm[2];
+      E.pl[f].w = (R)off;
+    }
+    E.opt.x = (R)m.mua[e];
+    E.opt.y = (R)m.mus[e];
+    E.opt.z = (R)m.n[e];
+    E.opt.w = (R)m.g[e];
+    E.nb = m.nbcode[e];
+  }
+}
+
+void build_nb_codes(vmc_problem &m) {
+  m.nbcode.assign(m.Ne, i4{kNbNone, kNbNone, kNbNone, kNbNone});
+  for (int64_t e = 0; e < m.Ne; ++e) {
+    int code[4] = {kNbNone, kNbNone, kNbNone, kNbNone};
+    for (int f = 0; f < m.nv; ++f) {
+      const int64_t v = m.hn(e, f);
+      if (v == kUnset) continue;
+      if (v < 0) {
+        if (-1 - v < m.Nb) code[f] = (int)v;
+        continue;
+      }
+      if (v >= m.Ne) continue;
+      int entry = 0, back = 0;
+      for (int q = 0; q < m.nv; ++q)
+        if (m.hn(v, q) == e) { entry = q; back = 1; break; }   // MC3D.hpp:1616-1628
+      code[f] = (int)((v << 3) | (back << 2) | entry);
+    }
+    m.nbcode[e] = i4{code[0], code[1], code[2], code[3]};
+  }
+}
+
+void build_boundary(vmc_problem &m) {
+  m.bnd.assign(m.Nb, BndRec{});
+  for (int64_t i = 0; i < m.Nb; ++i) {
+    const char t = m.BCType[i];
+    BndRec &b = m.bnd[i];
+    b.bcn = m.BCn[i];
+    if (t == 'm' || t == 'L' || t == 'I' || t == 'C') b.kind = kBndMirror;   // MC3D.hpp:1541
+    else b.kind = m.BCn[i] > 0.0 ? kBndFresnel : kBndAbsorb;                  // MC3D.hpp:1552
+  }
+}
+
+void upload_params(vmc_problem &m, bool f64) {
+  if (f64 && !m.params_dirty64) return;
+  if (!f64 && !m.params_dirty32) return;
+  if (m.dim == 3) {
+    if (f64) {
+      std::vector<Elem<4, double>> v;
+      pack_elements<4, double>(m, v);
+      m.elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    } else {
+      std::vector<Elem<4, float>> v;
+      pack_elements<4, float>(m, v);
+      m.elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    }
+  } else {
+    if (f64) {
+      std::vector<Elem<3, double>> v;
+      pack_elements<3, double>(m, v);
+      m.elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    } else {
+      std::vector<Elem<3, float>> v;
+      pack_elements<3, float>(m, v);
+      m.elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    }
+  }
+  (f64 ? m.params_dirty64 : m.params_dirty32) = false;
+}
+
+// Normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703)
+void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t N, double *ER, double *EI,
+               double *EBR, double *EBI, double *DEBR, double *DEBI) {
+  const bool d3 = m.dim == 3;
+  if (omega <= 0.0) {
+    for (int64_t i = 0; i < m.Ne; ++i) {
+      if (m.mua[i] > 0.0) ER[i] /= m.mua[i] * m.vol[i] * (double)N;
+      else ER[i] /= m.vol[i] * (double)N;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) EBR[i] /= (double)N * m.area[i];
+    if (!d3)
+      for (int64_t i = 0; i < m.Nb; ++i) DEBR[i] /= (double)N * m.area[i];
+  } else {
+    for (int64_t i = 0; i < m.Ne; ++i) {
+      const double a = ER[i], b = EI[i], k = kfac_n[i];
+      ER[i] = (b * k + a * m.mua[i]) / (pow(k, 2) + pow(m.mua[i], 2)) / (double)N / m.vol[i];
+      EI[i] = -(a * k - b * m.mua[i]) / (pow(k, 2) + pow(m.mua[i], 2)) / (double)N / m.vol[i];
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      EBR[i] /= (double)N * m.area[i];
+      EBI[i] /= (double)N * m.area[i];
+      if (!d3) {
+        DEBR[i] /= (double)N * m.area[i];
+        DEBI[i] /= (double)N * m.area[i];
+      }
+    }
+  }
+}
+
+int fail_hip(const HipFail &f) {
+  g_last_error = std::string(f.what) + ": " + hipGetErrorString(f.e);
+  return f.e == hipErrorOutOfMemory ? VMC_ERR_OUT_OF_MEMORY : VMC_ERR_HIP;
+}
+
+template <class T> void copy_in(std::vector<T> &dst, const T *src, int64_t n) {
+  if (src) dst.assign(src, src + n);
+  else dst.clear();
+}
+
+} // namespace
+
+extern "C" {
+
+} // extern "C"
+
+namespace {
+
+// ErrorChecks() and the host part of Init(); returns a status.
+int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
+  if ((d->dim != 2 && d->dim != 3) || d->Ne <= 0 || d->Nb <= 0 || d->Nr <= 0 || !d->H || !d->BH || !d->r ||
+      !d->mua || !d->mus || !d->g || !d->n || !d->BCType)
+    return VMC_ERR_INVALID_ARGUMENT;
+  if (d->Ne >= kMaxElements || d->Nb >= INT32_MAX) return VMC_ERR_INVALID_ARGUMENT;
+  try {
+    m->dim = d->dim;
+    m->nv = d->dim + 1;
+    m->nbv = d->dim;
+    m->Ne = d->Ne; m->Nb = d->Nb; m->Nr = d->Nr;
+    copy_in(m->H, d->H, d->Ne * m->nv);
+    copy_in(m->HN, d->HN, d->Ne * m->nv);
+    copy_in(m->BH, d->BH, d->Nb * m->nbv);
+    copy_in(m->r, d->r, d->Nr * m->dim);
+    copy_in(m->mua, d->mua, d->Ne);
+    copy_in(m->mus, d->mus, d->Ne);
+    copy_in(m->g, d->g, d->Ne);
+    copy_in(m->n, d->n, d->Ne);
+    copy_in(m->BCType, d->BCType, d->Nb);
+    copy_in(m->BCn, d->BCn, d->Nb);
+    copy_in(m->BCLNormal, d->BCLNormal, d->Nb * m->dim);
+    copy_in(m->BCLDirType, d->BCLightDirectionType, d->Nb);
+    copy_in(m->BCIntensity, d->BCIntensity, d->Nb);
+    if (m->dim == 2) copy_in(m->GaussianSigma, d->GaussianSigma, d->Nb);
+
+    error_checks(*m);
+    build_neighbours(*m);
+    if (m->BCIntensity.empty()) m->BCIntensity.assign(m->Nb, 1.0);
+    if (m->BCLDirType.empty()) m->BCLDirType.assign(m->Nb, 'n');
+    build_light_sources(*m);
+    prepare_directions(*m);
+
+    for (int dd = 0; dd < m->dim; ++dd) {
+      double lo = m->x(0, dd), hi = lo;
+      for (int64_t i = 1; i < m->Nr; ++i) {
+        lo = std::min(lo, m->x(i, dd));
+        hi = std::max(hi, m->x(i, dd));
+      }
+      m->shift[dd] = 0.5 * (lo + hi);
+    }
+    build_source_records(*m);
+    build_nb_codes(*m);
+    build_boundary(*m);
+    m->vol.resize(m->Ne);
+    m->area.resize(m->Nb);
+    for (int64_t i = 0; i < m->Ne; ++i) m->vol[i] = m->dim == 3 ? volume3(*m, i) : area2(*m, i);
+    for (int64_t i = 0; i < m->Nb; ++i)
+      m->area[i] = m->dim == 3 ? heron3(*m, m->bh(i, 0), m->bh(i, 1), m->bh(i, 2)) : len2(*m, m->bh(i, 0), m->bh(i, 1));
+  } catch (McError e) {
+    return e.code + 1;
+  }
+  return VMC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsources) {
+  if (!d) return VMC_ERR_INVALID_ARGUMENT;
+  vmc_problem m;
+  const int st = host_prepare(d, &m);
+  if (st != VMC_OK) return st;
+  if (HN_out) std::memcpy(HN_out, m.HN.data(), sizeof(int64_t) * m.HN.size());
+  if (nsources) *nsources = (int64_t)m.src.size();
+  return VMC_OK;
+}
+
+int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
+  if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  vmc_problem *m = new vmc_problem();
+  const int st = host_prepare(d, m);
+  if (st != VMC_OK) {
+    delete m;
+    return st;
+  }
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      delete m;
+      g_last_error = "no HIP device";
+      return VMC_ERR_NO_DEVICE;
+    }
+    hip_check(hipGetDevice(&m->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, m->device), "hipGetDeviceProperties");
+    m->num_cus = prop.multiProcessorCount;
+    hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&m->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&m->ev1), "hipEventCreate");
+    m->cdf.ensure(m->lsCDF.size());
+    hip_check(hipMemcpy(m->cdf.p, m->lsCDF.data(), m->lsCDF.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    m->dsrc.ensure(m->src.size());
+    hip_check(hipMemcpy(m->dsrc.p, m->src.data(), m->src.size() * sizeof(SrcRec), hipMemcpyHostToDevice), "upload");
+    m->dbnd.ensure(m->bnd.size());
+    hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
+    m->ledger.ensure(8);
+    m->counters.ensure(8);
+  } catch (HipFail f) {
+    int st = fail_hip(f);
+    vmc_problem_destroy(m);
+    return st;
+  }
+  *out = m;
+  return VMC_OK;
+}
+
+int vmc_problem_set_medium(vmc_problem *m, const double *mua, const double *mus, const double *g,
+                           const double *n) {
+  if (!m) return VMC_ERR_INVALID_ARGUMENT;
+  if (mua) m->mua.assign(mua, mua + m->Ne);
+  if (mus) m->mus.assign(mus, mus + m->Ne);
+  if (g) m->g.assign(g, g + m->Ne);
+  if (n) m->n.assign(n, n + m->Ne);
+  m->params_dirty32 = m->params_dirty64 = true;
+  return VMC_OK;
+}
+
+int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) {
+  if (!m || !HN) return VMC_ERR_INVALID_ARGUMENT;
+  std::memcpy(HN, m->HN.data(), sizeof(int64_t) * m->HN.size());
+  return VMC_OK;
+}
+
+int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
+                    vmc_progress_fn progress, void *user) {
+  if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
+  if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
+  if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  const bool f64 = run->precision == VMC_FP64;
+  const bool fixed = run->tally == VMC_TALLY_FIXED64;
+  const double omega = 2.0 * M_PI * run->f;
+  const bool fd = omega > 0.0;
+  double phase0 = run->phase0;
+  if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
+  int block = run->block_size > 0 ? run->block_size : 256;
+  if (block % 64 != 0 || block > 256) return VMC_ERR_INVALID_ARGUMENT;
+  int nrep = run->replicas > 0 ? run->replicas : 8;
+  if (nrep > 64) return VMC_ERR_INVALID_ARGUMENT;
+  const int64_t N = run->Nphoton;
+  const size_t ne_slots = (size_t)m->Ne * (fd ? 2 : 1), nb_slots = (size_t)m->Nb * 4;
+  int aborted = 0;
+  float kernel_ms = 0.f;
+  int grid = 0;
+  double led[5] = {0, 0, 0, 0, 0};
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  std::vector<double> raw_e(ne_slots), raw_b(nb_slots);
+  try {
+    hip_check(hipSetDevice(m->device), "hipSetDevice");
+    upload_params(*m, f64);
+    int bpc = 0;
+    hip_check(walk_occupancy(m->dim, f64, fd, fixed, block, &bpc), "occupancy");
+    if (run->blocks_per_cu > 0) bpc = std::min(bpc > 0 ? bpc : 1, run->blocks_per_cu);
+    if (bpc < 1) bpc = 1;
+    grid = bpc * m->num_cus;
+    m->etally.ensure(ne_slots * nrep);
+    m->btally.ensure(nb_slots * nrep);
+    m->raw_e.ensure(ne_slots);
+    m->raw_b.ensure(nb_slots);
+    hip_check(launch_fill_u64(m->etally.p, ne_slots * nrep, 0ull, m->stream), "fill");
+    hip_check(launch_fill_u64(m->btally.p, nb_slots * nrep, 0ull, m->stream), "fill");
+    hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(m->ledger.p), 8, 0ull, m->stream), "fill");
+    hip_check(launch_fill_u64(m->counters.p, 8, 0ull, m->stream), "fill");
+
+    WalkParams P{};
+    P.elems = f64 ? (const void *)m->elems64.p : (const void *)m->elems32.p;
+    P.cdf = m->cdf.p;
+    P.src = m->dsrc.p;
+    P.bnd = m->dbnd.p;
+    P.etally = m->etally.p;
+    P.btally = m->btally.p;
+    P.ledger = m->ledger.p;
+    P.counters = m->counters.p;
+    P.ne = m->Ne;
+    P.nb = m->Nb;
+    P.nsrc = (int32_t)m->src.size();
+    P.nrep = nrep;
+    P.key0 = (uint32_t)run->seed;
+    P.key1 = (uint32_t)(run->seed >> 32);
+    P.tag = run->stream_tag;
+    P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
+    P.refill_min = std::min(std::max(run->refill_min, 1), 64);
+    P.chunk = 128;
+    P.phase0 = phase0;
+    P.cph0 = cos(phase0);
+    P.sph0 = sin(phase0);
+    P.kfac = omega / kC0;
+    P.weight0 = 0.001;   // MC3D.hpp:182
+    P.chance = 0.1;      // MC3D.hpp:183
+    P.fix_scale = 4294967296.0;
+
+    // photon ranges: one launch, or ~20 when a progress callback wants ticks
+    const int nseg = progress ? 20 : 1;
+    int64_t done = 0;
+    for (int sgm = 0; sgm < nseg && done < N; ++sgm) {
+      const int64_t end = (sgm == nseg - 1) ? N : (N * (sgm + 1)) / nseg;
+      if (end <= done) continue;
+      hip_check(launch_fill_u64(m->counters.p, 1, (unsigned long long)done, m->stream), "fill");
+      P.n_end = (uint64_t)end;
+      hip_check(hipEventRecord(m->ev0, m->stream), "event");
+      hip_check(launch_walk(m->dim, f64, fd, fixed, grid, block, m->stream, P), "walk");
+      hip_check(hipEventRecord(m->ev1, m->stream), "event");
+      hip_check(hipEventSynchronize(m->ev1), "sync");
+      float ms = 0.f;
+      hip_check(hipEventElapsedTime(&ms, m->ev0, m->ev1), "event");
+      kernel_ms += ms;
+      done = end;
+      if (progress && !progress(100.0 * (double)done / (double)(N > 0 ? N : 1), user)) {
+        aborted = 1;
+        break;
+      }
+    }
+    const double inv = fixed ? 1.0 / 4294967296.0 : 1.0;
+    hip_check(launch_reduce(fixed, m->etally.p, m->raw_e.p, ne_slots, nrep, inv, m->stream), "reduce");
+    hip_check(launch_reduce(fixed, m->btally.p, m->raw_b.p, nb_slots, nrep, inv, m->stream), "reduce");
+    hip_check(hipMemcpyAsync(raw_e.data(), m->raw_e.p, ne_slots * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
+    hip_check(hipMemcpyAsync(raw_b.data(), m->raw_b.p, nb_slots * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
+    hip_check(hipMemcpyAsync(led, m->ledger.p, 5 * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
+    hip_check(hipMemcpyAsync(cnt, m->counters.p, 4 * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
+    hip_check(hipStreamSynchronize(m->stream), "sync");
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  // split raw tallies and normalise with the reference's expressions
+  std::vector<double> ER(m->Ne), EI(m->Ne, 0.0), EBR(m->Nb), EBI(m->Nb), DEBR(m->Nb), DEBI(m->Nb), kn(m->Ne);
+  for (int64_t i = 0; i < m->Ne; ++i) {
+    ER[i] = fd ? raw_e[2 * i] : raw_e[i];
+    if (fd) EI[i] = raw_e[2 * i + 1];
+    kn[i] = omega / kC0 * m->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+  }
+  for (int64_t i = 0; i < m->Nb; ++i) {
+    EBR[i] = raw_b[4 * i];
+    EBI[i] = raw_b[4 * i + 1];
+    DEBR[i] = raw_b[4 * i + 2];
+    DEBI[i] = raw_b[4 * i + 3];
+  }
+  normalise(*m, omega, kn.data(), N, ER.data(), EI.data(), EBR.data(), EBI.data(), DEBR.data(), DEBI.data());
+  if (out) {
+    if (out->ER) std::memcpy(out->ER, ER.data(), 8 * m->Ne);
+    if (out->EI) std::memcpy(out->EI, EI.data(), 8 * m->Ne);
+    if (out->EBR) std::memcpy(out->EBR, EBR.data(), 8 * m->Nb);
+    if (out->EBI) std::memcpy(out->EBI, EBI.data(), 8 * m->Nb);
+    if (out->DEBR) std::memcpy(out->DEBR, DEBR.data(), 8 * m->Nb);
+    if (out->DEBI) std::memcpy(out->DEBI, DEBI.data(), 8 * m->Nb);
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->photons = N;
+    stats->steps = (int64_t)cnt[1];
+    stats->loss = (int64_t)cnt[2];
+    stats->rechecks = (int64_t)cnt[3];
+    stats->kernel_ms = kernel_ms;
+    for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
+    stats->grid_blocks = grid;
+    stats->block_size = block;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return aborted ? VMC_ERR_ABORTED : VMC_OK;
+}
+
+void vmc_problem_destroy(vmc_problem *m) {
+  if (!m) return;
+  m->elems32.release();
+  m->elems64.release();
+  m->cdf.release();
+  m->dsrc.release();
+  m->dbnd.release();
+  m->etally.release();
+  m->btally.release();
+  m->counters.release();
+  m->ledger.release();
+  m->raw_e.release();
+  m->raw_b.release();
+  if (m->ev0) (void)hipEventDestroy(m->ev0);
+  if (m->ev1) (void)hipEventDestroy(m->ev1);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *HN,
+                    const int64_t *BH, const double *r, const char *BCType, const double *BCIntensity,
+                    const char *BCLightDirectionType, const double *BCLNormal, const double *BCn,
+                    const double *mua, const double *mus, const double *g, const double *n, double f,
+                    double phase0, int64_t Nphoton, const double *GaussianSigma, uint64_t seed,
+                    int32_t precision, double *ER, double *EI, double *EBR, double *EBI, double *DEBR,
+                    double *DEBI, int64_t *HN_out, int64_t *loss) {
+  vmc_problem_desc d{};
+  d.dim = dim;
+  d.Ne = Ne; d.Nb = Nb; d.Nr = Nr;
+  d.H = H; d.HN = HN; d.BH = BH; d.r = r;
+  d.mua = mua; d.mus = mus; d.g = g; d.n = n;
+  d.BCType = BCType; d.BCn = BCn; d.BCLNormal = BCLNormal;
+  d.BCLightDirectionType = BCLightDirectionType;
+  d.BCIntensity = BCIntensity;
+  d.GaussianSigma = GaussianSigma;
+  vmc_problem *p = nullptr;
+  int st = vmc_problem_create(&d, &p);
+  if (st != VMC_OK) return st;
+  vmc_run_desc rd{};
+  rd.f = f;
+  rd.phase0 = phase0;
+  rd.Nphoton = Nphoton;
+  rd.seed = seed;
+  rd.precision = precision;
+  rd.tally = VMC_TALLY_FP64;
+  vmc_output o{ER, EI, EBR, EBI, DEBR, DEBI};
+  vmc_stats s{};
+  st = vmc_problem_run(p, &rd, &o, &s, nullptr, nullptr);
+  if (st == VMC_OK && HN_out) vmc_problem_get_hn(p, HN_out);
+  if (loss) *loss = s.loss;
+  vmc_problem_destroy(p);
+  return st;
+}
+
+const char *vmc_status_string(int st) {
+  static const char *mc[] = {
+      // cpp/Errors.hpp:24-59 wording
+      "No lightsource",
+      "Scattering anisotropy array is not equal in size to the number of elements",
+      "Absorption array is not equal in size to the number of elements",
+      "Scattering coefficient array is not equal in size to the number of elements",
+      "Refractive index array is not equal in size to the number of elements",
+      "External refractive index array is not equal in size to the number of boundary elements",
+      "No external refractive index array given",
+      "The boundary condition array is not equal in size to the number of boundary elements",
+      "The light direction array is not equal in size to the number of boundary elements",
+      "The light direction type array is not equal in size to the number of boundary elements",
+      "H matrix contain indices that are not in the coordinate matrix",
+      "BH matrix contain indices that are not in the coordinate matrix",
+      "Ill defined mesh - cannot find nodes. Check Mesh.",
+      "Ill defined mesh - some boundary elements are trapped between two elements. A boundary element must belong to a single element. Check Mesh.",
+      "A gaussian light source exists but no sigma parameter was provided",
+      "Ill defined mesh - a complete boundary cannot be constructed. Check Mesh.",
+  };
+  if (st == VMC_OK) return "Success";
+  if (st >= 1 && st <= 16) return mc[st - 1];
+  switch (st) {
+  case VMC_ERR_UNDEFINED_DIRECTION:
+    return "A light source of type L, I or p needs a light direction (type a, r or p)";
+  case VMC_ERR_INVALID_ARGUMENT: return "Invalid argument";
+  case VMC_ERR_HIP: return "HIP runtime error";
+  case VMC_ERR_OUT_OF_MEMORY: return "Out of device memory";
+  case VMC_ERR_NO_DEVICE: return "No GPU device";
+  case VMC_ERR_ABORTED: return "Computation aborted by the progress callback";
+  default: return "Undefined error";
+  }
+}
+
+const char *vmc_last_error(void) { return g_last_error.c_str(); }
+
+int vmc_device_info(char *buf, int32_t buflen, int32_t *num_cus) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VMC_ERR_NO_DEVICE;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return VMC_ERR_HIP;
+  if (buf && buflen > 0)
+    snprintf(buf, (size_t)buflen, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return VMC_OK;
+}
+
+const char *vmc_version(void) { return "valomc-mi355x 0.1 (gfx950)"; }
+
+} // extern "C"

@@ -1,0 +1,721 @@
+// Photon-packet transport kernels for gfx950 (MI355X).
+//
+// One lane carries one photon packet from creation at a light source to its
+// end (exit, roulette, loss).  Waves are persistent: a wave whose lanes have
+// finished takes the next photon ids from a wave-private pool that is filled
+// from one global counter in chunks, so the walk keeps all 64 lanes busy
+// until the photon budget runs out.  Photon i draws its random numbers from
+// its own Philox4x32-10 stream keyed by (seed, tag, i), so a result does not
+// depend on the launch shape or on which lane ran which photon.
+//
+// Transport follows ValoMC's PropagatePhoton (cpp/3d/MC3D.hpp:1450-1638,
+// cpp/2d/MC2D.hpp:1307-1511) step for step and consumes random numbers in
+// the same order.  The exit face is found from precomputed outward face
+// planes (argmin of the positive plane distances) instead of the reference's
+// Moeller-Trumbore test on gathered vertices (MC3D.hpp:1006-1057,
+// 1825-1869); for a photon inside a convex element both pick the same face.
+//
+// Tallies go to `nrep` replicas selected by blockIdx % nrep, so consecutive
+// workgroups (which the dispatcher spreads over the XCDs) add into
+// different copies and hot elements near a source are not serialised on a
+// single L2 line; the replicas are summed after the walk.
+#include "vmc_types.h"
+
+#include <type_traits>
+
+namespace vmc {
+
+// ---------------------------------------------------------------------------
+// Random numbers
+// ---------------------------------------------------------------------------
+
+// Philox4x32-10 (Salmon et al., SC'11) with the published constants.
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t &o0,
+                                              uint32_t &o1, uint32_t &o2, uint32_t &o3) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  o0 = c0;
+  o1 = c1;
+  o2 = c2;
+  o3 = c3;
+}
+
+// Draw k of photon p is word k%4 of block (k/4, tag, p_lo, p_hi).
+struct Stream {
+  uint32_t plo, phi, k, b1, b2, b3;
+  __device__ __forceinline__ uint32_t next(const WalkParams &P) {
+    uint32_t r;
+    if ((k & 3u) == 0) {
+      philox4x32_10(k >> 2, P.tag, plo, phi, P.key0, P.key1, r, b1, b2, b3);
+    } else {
+      r = b1;
+      b1 = b2;
+      b2 = b3;
+    }
+    ++k;
+    return r;
+  }
+};
+
+// Uniform maps.  FP64 uses the reference's maps on the full 32-bit draw
+// (cpp/3d/mt_rng.hpp:186-204); FP32 uses the top 24 bits so every value is
+// exactly representable.
+template <class R> struct Unif;
+template <> struct Unif<double> {
+  static __device__ __forceinline__ double closed(uint32_t u) { return (double)u * (1.0 / 4294967295.0); }
+  static __device__ __forceinline__ double open(uint32_t u) { return ((double)u + 0.5) * (1.0 / 4294967296.0); }
+  static __device__ __forceinline__ double halfup(uint32_t u) { return (double)u * (1.0 / 4294967296.0); }
+};
+template <> struct Unif<float> {
+  static __device__ __forceinline__ float closed(uint32_t u) {
+    return fminf((float)(u >> 8) * (1.0f / 16777215.0f), 1.0f);
+  }
+  static __device__ __forceinline__ float open(uint32_t u) { return ((float)(u >> 8) + 0.5f) * 5.9604644775390625e-8f; }
+  static __device__ __forceinline__ float halfup(uint32_t u) { return (float)(u >> 8) * 5.9604644775390625e-8f; }
+};
+
+// ---------------------------------------------------------------------------
+// Precision-generic math
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float m_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float m_log(float x) { return logf(x); }
+__device__ __forceinline__ double m_log(double x) { return log(x); }
+__device__ __forceinline__ float m_exp(float x) { return expf(x); }
+__device__ __forceinline__ double m_exp(double x) { return exp(x); }
+__device__ __forceinline__ float m_expm1(float x) { return expm1f(x); }
+__device__ __forceinline__ double m_expm1(double x) { return expm1(x); }
+__device__ __forceinline__ float m_asin(float x) { return asinf(x); }
+__device__ __forceinline__ double m_asin(double x) { return asin(x); }
+__device__ __forceinline__ float m_erf(float x) { return erff(x); }
+__device__ __forceinline__ double m_erf(double x) { return erf(x); }
+__device__ __forceinline__ void m_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void m_sincos(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void m_sincospi(float x, float *s, float *c) { sincospif(x, s, c); }
+__device__ __forceinline__ void m_sincospi(double x, double *s, double *c) { sincospi(x, s, c); }
+
+template <class R> struct Tol;
+template <> struct Tol<float> { static constexpr float out = 1e-6f; };
+template <> struct Tol<double> { static constexpr double out = 1e-12; };
+
+// ---------------------------------------------------------------------------
+// Tally adds
+// ---------------------------------------------------------------------------
+
+template <bool FIXED> __device__ __forceinline__ void tally_add(void *base, int64_t idx, double v, double scale);
+template <> __device__ __forceinline__ void tally_add<false>(void *base, int64_t idx, double v, double) {
+  unsafeAtomicAdd(reinterpret_cast<double *>(base) + idx, v);
+}
+template <> __device__ __forceinline__ void tally_add<true>(void *base, int64_t idx, double v, double scale) {
+  const long long q = __double2ll_rn(v * scale);
+  atomicAdd(reinterpret_cast<unsigned long long *>(base) + idx, (unsigned long long)q);
+}
+
+// ---------------------------------------------------------------------------
+// Photon state and helpers
+// ---------------------------------------------------------------------------
+
+template <class R> struct Photon {
+  R x[3], d[3];
+  R w, prop;
+  R cph, sph;   // cos/sin of the phase (frequency domain)
+  int el, face, nsteps;
+};
+
+// Load the record of element `el` into registers.
+template <int NF, class R>
+__device__ __forceinline__ void load_elem(const Elem<NF, R> *__restrict__ E, int el,
+                                          typename Vec4T<R>::type (&pl)[NF],
+                                          typename Vec4T<R>::type &opt, i4 &nb) {
+  const Elem<NF, R> *e = E + el;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) pl[f] = e->pl[f];
+  opt = e->opt;
+  nb = e->nb;
+}
+
+// Fresnel transmission/reflection at a face with outward unit normal n
+// (MC3D::FresnelPhoton, MC3D.hpp:1365-1447).  The reflection probability is
+// the unpolarised Fresnel formula in cosine form, which equals the
+// reference's sin/tan form given Snell's law, including its normal-incidence
+// limit.  Returns true when the packet is reflected.
+template <int D, class R>
+__device__ __forceinline__ bool fresnel(Photon<R> &p, const R n[3], R n1, R n2, bool refract,
+                                        Stream &rs, const WalkParams &P) {
+  const R eta = n1 / n2;
+  R ci = p.d[0] * n[0] + p.d[1] * n[1];
+  if (D == 3) ci += p.d[2] * n[2];
+  const R s2t = eta * eta * (R(1) - ci * ci);
+  bool refl = true;
+  if (s2t < R(1)) {
+    const R ct = m_sqrt(R(1) - s2t);
+    const R rs_ = (eta * ci - ct) / (eta * ci + ct);
+    const R rp = (ci - eta * ct) / (ci + eta * ct);
+    R Rf = R(0.5) * (rs_ * rs_ + rp * rp);
+    // a direction longer than unit length (an unnormalised absolute light
+    // direction before its first scattering) makes the reference's
+    // acos(costhi) NaN, and its sin(thi + tht) > eps test then selects the
+    // normal-incidence value (MC3D.hpp:1409-1419)
+    if (ci > R(1)) Rf = ((eta - R(1)) / (eta + R(1))) * ((eta - R(1)) / (eta + R(1)));
+    refl = Unif<R>::closed(rs.next(P)) <= Rf;
+    if (!refl && refract) {
+      const R s = ct - eta * ci;
+#pragma unroll
+      for (int k = 0; k < D; ++k) p.d[k] = eta * p.d[k] + s * n[k];
+    }
+  }
+  if (refl) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) p.d[k] -= R(2) * ci * n[k];
+  }
+  return refl;
+}
+
+// Henyey-Greenstein scattering, MC3D::ScatterPhoton (MC3D.hpp:1297-1340).
+template <class R>
+__device__ __forceinline__ void scatter3(Photon<R> &p, R g, Stream &rs, const WalkParams &P) {
+  R cost;
+  if (g != R(0)) {
+    const R xi = Unif<R>::closed(rs.next(P));
+    if (R(0) < xi && xi < R(1)) {
+      const R g2 = g * g;
+      const R t = (R(1) - g2) / (R(1) - g * (R(1) - R(2) * xi));
+      cost = (R(1) + g2 - t * t) / (R(2) * g);
+    } else {
+      cost = xi >= R(1) ? R(1) : R(-1);
+    }
+  } else {
+    cost = R(2) * Unif<R>::closed(rs.next(P)) - R(1);
+  }
+  cost = fmin(fmax(cost, R(-1)), R(1));
+  const R sint = m_sqrt(fmax(R(1) - cost * cost, R(0)));
+  R sp, cp;
+  m_sincospi(R(2) * Unif<R>::closed(rs.next(P)), &sp, &cp);
+  R nx, ny, nz;
+  if (fabs(p.d[2]) > R(0.999)) {
+    nx = sint * cp;
+    ny = sint * sp;
+    nz = p.d[2] >= R(0) ? cost : -cost;
+  } else {
+    const R tmp = m_sqrt(R(1) - p.d[2] * p.d[2]);
+    const R itmp = R(1) / tmp;
+    nx = sint * (p.d[0] * p.d[2] * cp - p.d[1] * sp) * itmp + p.d[0] * cost;
+    ny = sint * (p.d[1] * p.d[2] * cp + p.d[0] * sp) * itmp + p.d[1] * cost;
+    nz = -sint * cp * tmp + p.d[2] * cost;
+  }
+  const R inv = R(1) / m_sqrt(nx * nx + ny * ny + nz * nz);
+  p.d[0] = nx * inv;
+  p.d[1] = ny * inv;
+  p.d[2] = nz * inv;
+  p.face = -1;
+}
+
+// 2D Henyey-Greenstein, MC2D::ScatterPhoton (MC2D.hpp:1135-1155):
+// theta = 2 atan(tan(pi xi) (1-g)/(1+g)), applied as a rotation.
+template <class R>
+__device__ __forceinline__ void scatter2(Photon<R> &p, R g, Stream &rs, const WalkParams &P) {
+  const R xi = Unif<R>::halfup(rs.next(P));
+  R s, c;
+  if (g != R(0)) {
+    m_sincospi(xi, &s, &c);
+    const R q = (R(1) - g) / (R(1) + g);
+    const R a = q * s;
+    const R den = R(1) / (c * c + a * a);
+    const R ct = (c * c - a * a) * den, st = R(2) * a * c * den;
+    // the reference rebuilds the direction from its angle, so the result
+    // is a unit vector whatever the input length
+    const R il = R(1) / m_sqrt(p.d[0] * p.d[0] + p.d[1] * p.d[1]);
+    const R dx = p.d[0] * il, dy = p.d[1] * il;
+    p.d[0] = dx * ct - dy * st;
+    p.d[1] = dx * st + dy * ct;
+  } else {
+    m_sincospi(R(2) * xi, &s, &c);
+    p.d[0] = c;
+    p.d[1] = s;
+  }
+  p.face = -1;
+}
+
+// Truncated-normal launch angle by the reference's 16-step bisection of the
+// CDF (MC2D.hpp:1044-1056).
+template <class R> __device__ __forceinline__ R gauss_angle(R xi, R sig, R h) {
+  const R pi = R(3.14159265358979323846);
+  R a = -pi / R(2), b = pi / R(2), phi = R(0);
+  const R inv = R(1) / (sig * R(1.41421356237309504880));
+  for (int it = 0; it < 16; ++it) {
+    phi = (a + b) / R(2);
+    const R cdf = (R(0.5) + R(0.5) * m_erf(phi * inv) - h) / (R(1) - R(2) * h);
+    if (cdf - xi < R(0)) a = phi; else b = phi;
+  }
+  return phi;
+}
+
+// CreatePhoton (MC3D.hpp:1060-1294, MC2D.hpp:958-1132).  Returns the source's
+// mother element and initial face.
+template <int D, class R>
+__device__ __forceinline__ void create_photon(Photon<R> &p, Stream &rs, const WalkParams &P) {
+  // source entry: first CDF value above xi, clamped (MC2D.hpp:966-969)
+  const double xi = Unif<double>::closed(rs.next(P));
+  int lo = 0, hi = P.nsrc;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (xi < P.cdf[mid]) hi = mid; else lo = mid + 1;
+  }
+  const SrcRec &s = P.src[lo < P.nsrc ? lo : P.nsrc - 1];
+  p.el = s.mother;
+  p.face = s.face;
+  const int kind = s.kind;
+  if constexpr (D == 3) {
+    const R r1 = Unif<R>::open(rs.next(P)), r2 = Unif<R>::open(rs.next(P));
+    const R sr = m_sqrt(r1);
+    const R c0 = R(1) - sr, c1 = sr * (R(1) - r2), c2 = sr * r2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.x[k] = c0 * R(s.v0[k]) + c1 * R(s.v1[k]) + c2 * R(s.v2[k]);
+    const R n0 = R(s.nin[0]), n1 = R(s.nin[1]), n2 = R(s.nin[2]);
+    if (kind == kSrcGiven) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p.d[k] = R(s.dir[k]);
+    } else if (kind == kSrcPencil) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        p.d[k] = R(s.nin[k]);
+        p.x[k] = R(s.dir[k]);
+      }
+    } else if (kind == kSrcIso3) {
+      R v0 = n0, v1 = n1, v2 = n2;
+      for (int it = 0; it < 256; ++it) {
+        const R th = Unif<R>::halfup(rs.next(P));
+        const R u = R(2) * Unif<R>::closed(rs.next(P)) - R(1);
+        R st, ct;
+        m_sincospi(R(2) * th, &st, &ct);
+        const R q = m_sqrt(fmax(R(1) - u * u, R(0)));
+        v0 = q * ct;
+        v1 = q * st;
+        v2 = u;
+        if (n0 * v0 + n1 * v1 + n2 * v2 > R(0)) break;
+      }
+      p.d[0] = v0;
+      p.d[1] = v1;
+      p.d[2] = v2;
+    } else if (kind == kSrcCos3) {
+      const R ph = m_asin(R(2) * Unif<R>::open(rs.next(P)) - R(1));
+      const R th = Unif<R>::closed(rs.next(P));
+      R sp, cp, st, ct;
+      m_sincos(ph, &sp, &cp);
+      m_sincospi(R(2) * th, &st, &ct);
+      const R f1 = ct * sp, f2 = st * sp;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) p.d[k] = cp * R(s.nin[k]) + f1 * R(s.a1[k]) + f2 * R(s.a2[k]);
+    } else {
+      p.d[0] = n0;
+      p.d[1] = n1;
+      p.d[2] = n2;
+    }
+  } else {
+    const R w = Unif<R>::open(rs.next(P));
+#pragma unroll
+    for (int k = 0; k < 2; ++k) p.x[k] = R(s.v0[k]) + w * (R(s.v1[k]) - R(s.v0[k]));
+    p.x[2] = R(0);
+    p.d[2] = R(0);
+    const R n0 = R(s.nin[0]), n1 = R(s.nin[1]);
+    const R phi0 = R(s.phi0);
+    if (kind == kSrcNormal) {
+      p.d[0] = n0;
+      p.d[1] = n1;
+    } else if (kind == kSrcGiven) {
+      p.d[0] = R(s.dir[0]);
+      p.d[1] = R(s.dir[1]);
+    } else {
+      for (int it = 0; it < 256; ++it) {
+        R phi;
+        if (kind == kSrcIso2) {
+          phi = R(3.14159265358979323846) * (Unif<R>::open(rs.next(P)) - R(0.5));
+        } else if (kind == kSrcCos2 || kind == kSrcCos2Rej) {
+          phi = m_asin(R(2) * Unif<R>::open(rs.next(P)) - R(1));
+        } else {
+          phi = gauss_angle<R>(Unif<R>::open(rs.next(P)), R(s.gsig), R(s.gh));
+        }
+        R sn, cs;
+        m_sincos(phi0 + phi, &sn, &cs);
+        p.d[0] = cs;
+        p.d[1] = sn;
+        if (!(kind == kSrcCos2Rej || kind == kSrcGauss2Rej)) break;
+        if (n0 * cs + n1 * sn > R(0)) break;
+      }
+    }
+  }
+  p.w = R(1);
+  p.cph = R(P.cph0);
+  p.sph = R(P.sph0);
+  p.nsteps = 0;
+}
+
+// ---------------------------------------------------------------------------
+// The walk
+// ---------------------------------------------------------------------------
+
+template <int D, class R, bool FD, bool FIXED>
+__global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
+  constexpr int NF = D + 1;
+  typedef typename Vec4T<R>::type V4;
+  const Elem<NF, R> *__restrict__ E = reinterpret_cast<const Elem<NF, R> *>(P.elems);
+  const int rep = (int)(blockIdx.x % (unsigned)P.nrep);
+  void *const etally = reinterpret_cast<char *>(P.etally) + (size_t)rep * P.ne * (FD ? 2 : 1) * 8;
+  void *const btally = reinterpret_cast<char *>(P.btally) + (size_t)rep * P.nb * 4 * 8;
+  const int lane = (int)__lane_id();
+
+  Photon<R> p;
+  Stream rs;
+  bool alive = false;
+  double l_abs = 0, l_exit = 0, l_kill = 0, l_gain = 0, l_lost = 0;
+  uint32_t steps = 0, losses = 0;
+
+  // wave-private photon id pool [pool_next, pool_end)
+  uint64_t pool_next = 0, pool_end = 0;
+  bool exhausted = false;
+  const int refill_min = P.refill_min;
+
+  for (;;) {
+    if (!exhausted) {
+      const uint64_t dead = __ballot(!alive);
+      const int ndead = __popcll(dead);
+      if (ndead >= refill_min) {
+        const uint64_t avail = pool_end - pool_next;
+        uint64_t fresh = 0;
+        if (avail < (uint64_t)ndead) {
+          unsigned long long got = 0;
+          if (lane == 0) got = atomicAdd(&P.counters[0], (unsigned long long)P.chunk);
+          const uint32_t glo = __builtin_amdgcn_readfirstlane((uint32_t)got);
+          const uint32_t ghi = __builtin_amdgcn_readfirstlane((uint32_t)(got >> 32));
+          fresh = ((uint64_t)ghi << 32) | glo;
+        }
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
+        uint64_t pid = 0;
+        if (!alive) pid = (uint64_t)rank < avail ? pool_next + rank : fresh + (rank - avail);
+        if (avail < (uint64_t)ndead) {
+          pool_next = fresh + (ndead - avail);
+          pool_end = fresh + (uint64_t)P.chunk;
+        } else {
+          pool_next += ndead;
+        }
+        const bool invalid = !alive && pid >= P.n_end;
+        if (__ballot(invalid)) exhausted = true;
+        if (!alive && !invalid) {
+          rs.plo = (uint32_t)pid;
+          rs.phi = (uint32_t)(pid >> 32);
+          rs.k = 0;
+          create_photon<D, R>(p, rs, P);
+          p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
+          alive = true;
+        }
+      }
+    }
+    if (__ballot(alive) == 0) break;
+    if (!alive) continue;
+
+    // ---- one element step ----
+    // the element record is fetched per step rather than carried across
+    // iterations: it keeps the loop state small (more waves per SIMD) and a
+    // repeated fetch of the same record hits in the vector L1
+    V4 pl[NF];
+    V4 opt;
+    i4 nb;
+    load_elem<NF, R>(E, p.el, pl, opt, nb);
+    ++steps;
+    ++p.nsteps;
+    // exit face: smallest positive plane distance over faces moving outward,
+    // the current face excluded (MC3D.hpp:1020-1056)
+    R bn = R(1), bc = R(0);
+    int bf = -1, code = kNbNone;
+    R nrm[3] = {R(0), R(0), R(0)};
+    bool out_cur = false;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      R c = pl[f].x * p.d[0] + pl[f].y * p.d[1];
+      R q = pl[f].x * p.x[0] + pl[f].y * p.x[1];
+      if (D == 3) {
+        c += pl[f].z * p.d[2];
+        q += pl[f].z * p.x[2];
+      }
+      const R num = fmax(pl[f].w - q, R(0));
+      const bool cur = f == p.face;
+      out_cur = cur ? c > Tol<R>::out : out_cur;
+      if (!cur && c > R(0) && num * bc < bn * c) {
+        bn = num;
+        bc = c;
+        bf = f;
+        code = nb[f];
+        nrm[0] = pl[f].x;
+        nrm[1] = pl[f].y;
+        nrm[2] = D == 3 ? pl[f].z : R(0);
+      }
+    }
+    if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
+      // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
+      l_lost += (double)p.w;
+      ++losses;
+      alive = false;
+      continue;
+    }
+    const R dist = bn / bc;
+    const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
+#pragma unroll
+    for (int k = 0; k < D; ++k) p.x[k] += p.d[k] * ds;
+
+    const R mua = opt.x, mus = opt.y, nidx = opt.z;
+    // element tally and attenuation (MC3D.hpp:1479-1526)
+    constexpr bool kF64 = std::is_same<R, double>::value;
+    // FP64: the reference's att = exp(-mua ds) and (1 - att) w.  FP32: the
+    // deposit -w expm1(-mua ds) is subtracted from w exactly, so the weight
+    // ledger closes to rounding of its FP64 sums.
+    const R em = kF64 ? R(0) : m_expm1(-mua * ds);
+    const R att = kF64 ? m_exp(-mua * ds) : R(1) + em;
+    const R dabs = kF64 ? p.w - p.w * att : -p.w * em;
+    if constexpr (!FD) {
+      const R dep = mua > R(0) ? (kF64 ? (R(1) - att) * p.w : dabs) : p.w * ds;
+      if (dep != R(0)) tally_add<FIXED>(etally, p.el, (double)dep, P.fix_scale);
+    } else {
+      // w e^{-i phi} (1 - e^{-(mua + i k) ds}), MC3D.hpp:1519-1522, with the
+      // phase advance b = k ds taken from half-angle terms
+      const R b = R(P.kfac) * nidx * ds;
+      R sh, ch;
+      m_sincos(R(0.5) * b, &sh, &ch);
+      const R sb = R(2) * sh * ch, ver = R(2) * sh * sh, cb = R(1) - ver;
+      const R c1 = p.cph * cb - p.sph * sb, s1 = p.sph * cb + p.cph * sb;
+      R er, ei;
+      if constexpr (kF64) {
+        er = p.w * (p.cph - c1 * att);
+        ei = p.w * (-p.sph + s1 * att);
+      } else {
+        const R qq = -em + att * ver;   // 1 - att cos(b) without cancellation
+        er = p.w * (p.cph * qq + att * p.sph * sb);
+        ei = p.w * (-p.sph * qq + att * p.cph * sb);
+      }
+      tally_add<FIXED>(etally, 2 * (int64_t)p.el, (double)er, P.fix_scale);
+      tally_add<FIXED>(etally, 2 * (int64_t)p.el + 1, (double)ei, P.fix_scale);
+      if constexpr (kF64) {
+        p.cph = c1;
+        p.sph = s1;
+      } else {
+        const R rn = R(1.5) - R(0.5) * (c1 * c1 + s1 * s1);   // keep |e^{i phi}| = 1
+        p.cph = c1 * rn;
+        p.sph = s1 * rn;
+      }
+    }
+    l_abs += (double)dabs;
+    p.w = kF64 ? p.w * att : p.w - dabs;
+    p.prop -= ds;
+
+    if (p.prop <= R(0)) {
+      // scattering event (MC3D.hpp:1634-1636), then a new free path
+      if (mus > R(0)) {
+        if constexpr (D == 3) scatter3<R>(p, opt.w, rs, P);
+        else scatter2<R>(p, opt.w, rs, P);
+      }
+      p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus;
+      continue;
+    }
+
+    if (code < 0) {
+      if (code == kNbNone) {
+        l_lost += (double)p.w;
+        ++losses;
+        alive = false;
+        continue;
+      }
+      // boundary (MC3D.hpp:1536-1567)
+      const int ib = -1 - code;
+      const BndRec b = P.bnd[ib];
+      if (b.kind == kBndMirror) {
+        R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
+        if (D == 3) cd += p.d[2] * nrm[2];
+#pragma unroll
+        for (int k = 0; k < D; ++k) p.d[k] -= R(2) * cd * nrm[k];
+        p.face = bf;
+        continue;
+      }
+      if (b.kind == kBndFresnel && fresnel<D, R>(p, nrm, nidx, R(b.bcn), false, rs, P)) {
+        p.face = bf;
+        continue;
+      }
+      const size_t bo = (size_t)ib * 4;
+      if (!FD) {
+        tally_add<FIXED>(btally, bo, (double)p.w, P.fix_scale);
+      } else {
+        tally_add<FIXED>(btally, bo, (double)(p.w * p.cph), P.fix_scale);
+        tally_add<FIXED>(btally, bo + 1, (double)(-p.w * p.sph), P.fix_scale);
+      }
+      if constexpr (D == 2) {
+        // exitance weighted by 1/(dir . outward normal), MC2D.hpp:1410-1437
+        const R dn = p.d[0] * nrm[0] + p.d[1] * nrm[1];
+        if (!FD) {
+          tally_add<FIXED>(btally, bo + 2, (double)(p.w / dn), P.fix_scale);
+        } else {
+          tally_add<FIXED>(btally, bo + 2, (double)(p.w * p.cph / dn), P.fix_scale);
+          tally_add<FIXED>(btally, bo + 3, (double)(-p.w * p.sph / dn), P.fix_scale);
+        }
+      }
+      l_exit += (double)p.w;
+      alive = false;
+      continue;
+    }
+
+    // interior face: enter the neighbour (MC3D.hpp:1570-1631)
+    const int next = code >> 3;
+    const bool back = (code >> 2) & 1;
+    const int entry = code & 3;
+    const R mus_c = mus, n_c = nidx;
+    const V4 opt_n = E[next].opt;
+    const R mus_n = opt_n.y, n_n = opt_n.z;
+    bool reflected = false;
+    if (mus_c <= R(0) && mus_n > R(0)) {
+      // vacuum -> scattering medium: fresh free path
+      p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus_n;
+    }
+    if (p.w < R(P.weight0)) {
+      // Russian roulette at the crossing (MC3D.hpp:1597-1604)
+      if (Unif<R>::closed(rs.next(P)) > R(P.chance)) {
+        l_kill += (double)p.w;
+        alive = false;
+        continue;
+      }
+      const R wn = p.w / R(P.chance);
+      l_gain += (double)wn - (double)p.w;
+      p.w = wn;
+    }
+    if (n_c != n_n) reflected = fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P);
+    if (reflected) {
+      p.face = bf;
+      continue;
+    }
+    if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= mus_c / mus_n;
+    if (!back) {
+      // the neighbour does not list this element: back-lookup failure
+      l_lost += (double)p.w;
+      ++losses;
+      alive = false;
+      continue;
+    }
+    p.el = next;
+    p.face = entry;
+  }
+
+  // ---- flush per-lane counters ----
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    l_abs += __shfl_xor(l_abs, off);
+    l_exit += __shfl_xor(l_exit, off);
+    l_kill += __shfl_xor(l_kill, off);
+    l_gain += __shfl_xor(l_gain, off);
+    l_lost += __shfl_xor(l_lost, off);
+    steps += __shfl_xor(steps, off);
+    losses += __shfl_xor(losses, off);
+  }
+  if (lane == 0) {
+    unsafeAtomicAdd(&P.ledger[0], l_abs);
+    unsafeAtomicAdd(&P.ledger[1], l_exit);
+    unsafeAtomicAdd(&P.ledger[2], l_kill);
+    unsafeAtomicAdd(&P.ledger[3], l_gain);
+    unsafeAtomicAdd(&P.ledger[4], l_lost);
+    atomicAdd(&P.counters[1], (unsigned long long)steps);
+    atomicAdd(&P.counters[2], (unsigned long long)losses);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Helper kernels
+// ---------------------------------------------------------------------------
+
+__global__ void fill_u64_kernel(unsigned long long *p, size_t n, unsigned long long v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Sum the tally replicas in a fixed order: out[i] = sum_r rep[r][i].
+template <bool FIXED>
+__global__ void reduce_replicas_kernel(const void *rep, double *out, size_t n, int nrep, double inv_scale) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (FIXED) {
+      const long long *r = reinterpret_cast<const long long *>(rep);
+      long long s = 0;
+      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * n + i];
+      out[i] = (double)s * inv_scale;
+    } else {
+      const double *r = reinterpret_cast<const double *>(rep);
+      double s = 0.0;
+      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * n + i];
+      out[i] = s;
+    }
+  }
+}
+
+} // namespace vmc
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (called from vmc_host.cpp)
+// ---------------------------------------------------------------------------
+
+namespace vmc {
+
+typedef void (*WalkFn)(const WalkParams);
+
+static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed) {
+#define VMC_PICK(D, R)                                                                  \
+  return fd ? (fixed ? walk_kernel<D, R, true, true> : walk_kernel<D, R, true, false>)   \
+            : (fixed ? walk_kernel<D, R, false, true> : walk_kernel<D, R, false, false>)
+  if (dim == 3) {
+    if (f64) { VMC_PICK(3, double); } else { VMC_PICK(3, float); }
+  } else {
+    if (f64) { VMC_PICK(2, double); } else { VMC_PICK(2, float); }
+  }
+#undef VMC_PICK
+}
+
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed), block, 0);
+}
+
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, hipStream_t s,
+                       const WalkParams &P) {
+  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed), dim3(grid), dim3(block), 0, s, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int block = 256;
+  const size_t want = (n + block - 1) / block;
+  const int grid = (int)(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid), dim3(block), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, double inv_scale,
+                         hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const int block = 256;
+  const size_t want = (n + block - 1) / block;
+  const int grid = (int)(want < 8192 ? want : 8192);
+  if (fixed)
+    hipLaunchKernelGGL(reduce_replicas_kernel<true>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, inv_scale);
+  else
+    hipLaunchKernelGGL(reduce_replicas_kernel<false>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, inv_scale);
+  return hipGetLastError();
+}
+
+} // namespace vmc

@@ -1,0 +1,99 @@
+// Internal host/device types of the transport engine.
+//
+// Element records are array-of-structs so one photon step touches one or two
+// cache lines: the face planes, the per-face neighbour codes and the optical
+// parameters of the element a photon is in are loaded together when it
+// enters the element.  Coordinates are shifted by the mesh centre so FP32
+// positions keep the most bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vmc {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef int i4 __attribute__((ext_vector_type(4)));
+
+template <class R> struct Vec4T;
+template <> struct Vec4T<float> { typedef f4 type; };
+template <> struct Vec4T<double> { typedef d4 type; };
+
+// Neighbour code of one element face (reference HN, cpp/3d/MC3D.hpp:1025,
+// 1616-1628, folded with the entry-face back-lookup):
+//   interior: (next << 3) | (back << 2) | entry   back = 1 when next lists
+//             this element among its own neighbours, entry = that face
+//   boundary: -1 - ib
+//   kNbNone:  face matched nothing (the reference leaves INT_FAST64_MAX)
+constexpr int32_t kNbNone = INT32_MIN;
+constexpr int64_t kMaxElements = (int64_t)1 << 28;
+
+// Face f: outward unit normal n and offset d with n.x <= d inside.
+// opt = (mua, mus, n, g).  Padding keeps records on 64-byte boundaries.
+template <int NF, class R> struct alignas(64) Elem {
+  typename Vec4T<R>::type pl[NF];
+  typename Vec4T<R>::type opt;
+  i4 nb;
+};
+
+// Boundary element kinds for the walk (BCType, MC3D.hpp:1536-1567)
+enum BndKind : int32_t {
+  kBndAbsorb = 0,   // leave the mesh (no Fresnel: BCn <= 0)
+  kBndFresnel = 1,  // Fresnel test against BCn, then leave
+  kBndMirror = 2,   // m, L, I, C: specular reflection
+};
+struct alignas(16) BndRec {
+  double bcn;
+  int32_t kind;
+  int32_t pad;
+};
+
+// How a light source launches photons (MC3D.hpp:1109-1287, MC2D.hpp:1004-1125)
+enum SrcKind : int32_t {
+  kSrcNormal = 0,    // along the inward normal
+  kSrcGiven = 1,     // along BCLNormal as given
+  kSrcPencil = 2,    // inward normal from the position in BCLNormal
+  kSrcIso3 = 3,      // uniform on the inward hemisphere (rejection)
+  kSrcCos3 = 4,      // phi = asin(2U-1) about frame (a0, a1, a2)
+  kSrcIso2 = 5,      // phi0 + pi(U - 1/2)
+  kSrcCos2 = 6,      // phi0 + asin(2U-1)
+  kSrcCos2Rej = 7,   // as kSrcCos2, redrawn until it points inwards
+  kSrcGauss2 = 8,    // phi0 + truncated-normal angle by 16-step bisection
+  kSrcGauss2Rej = 9, // as kSrcGauss2, redrawn until it points inwards
+};
+
+struct alignas(16) SrcRec {
+  double v0[3], v1[3], v2[3];  // triangle (2D: segment v0-v1), shifted
+  double nin[3];               // inward unit normal (centroid test)
+  double a1[3], a2[3];         // cosinic frame second and third axes
+  double dir[3];               // given direction, or pencil position (shifted)
+  double phi0, gsig, gh;       // 2D: base angle; Gaussian sigma and tail mass
+  int32_t mother, face, kind, ib;
+};
+
+// Tally layout: nrep replicas of [Ne][cw ? 1 : 2] element and [Nb][4]
+// boundary slots (EBR, EBI, DEBR, DEBI).
+struct WalkParams {
+  const void *elems;
+  const double *cdf;
+  const SrcRec *src;
+  const BndRec *bnd;
+  void *etally;
+  void *btally;
+  double *ledger;               // [5] absorbed, exited, killed, gained, lost
+  unsigned long long *counters; // [4] photon dispenser, steps, loss, rechecks
+  int64_t ne, nb;
+  uint64_t n_end;               // photons [*, n_end) of this run
+  int32_t nsrc, nrep;
+  uint32_t key0, key1, tag;
+  uint32_t step_cap;
+  int32_t refill_min;
+  int32_t chunk;
+  double phase0, cph0, sph0;    // initial phase and its cos/sin
+  double kfac;                  // omega / c0: k = kfac * n
+  double weight0, chance;
+  double fix_scale;             // fixed-point scale (2^32)
+};
+
+} // namespace vmc
