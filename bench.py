@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
+    ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
     return ap.parse_args()
 
 
@@ -84,7 +85,7 @@ def main():
     p = meshgen.sphere_box_3d(nphoton=nphot, seed=1234)
     prob = vmc.Problem(p)
     kw = dict(block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, refill_min=a.refill_min,
-              replicas=a.replicas)
+              replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
 
     for i in range(a.warmup):
         prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)

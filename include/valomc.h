@@ -73,13 +73,16 @@ typedef struct vmc_run_desc {
   uint64_t seed;         /* photon i of a run draws from the stream (seed, i)  */
   int32_t precision;     /* VMC_FP64 | VMC_FP32                                */
   int32_t tally;         /* VMC_TALLY_FP64 | VMC_TALLY_FIXED64                 */
-  int32_t block_size;    /* 0 = default (256)                                  */
+  int32_t block_size;    /* 0 = default (FP32 1024, FP64 256); multiple of 64   */
   int32_t blocks_per_cu; /* 0 = as many as fit                                 */
   int32_t replicas;      /* tally copies, 0 = one per XCD (8)                  */
   int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 1 */
   uint32_t stream_tag;   /* extra stream word: runs with different tags are
                             independent (batch instances use their index)      */
-  int32_t reserved[5];
+  int32_t lds_slots;     /* per-workgroup LDS tally cache entries (power of 2);
+                            0 = default (FP32 8192 CW / 4096 FD, FP64 4096 CW
+                            / 2048 FD), -1 = off, at most 16384                */
+  int32_t reserved[4];
 } vmc_run_desc;
 
 typedef struct vmc_output {

@@ -12,9 +12,10 @@
  *     remainder on the last thread, tallies reduced in thread order
  *     (MC3D.hpp:1648-1748).  Pinned bit-for-bit against the reference engine.
  *   ORACLE_RNG_PHILOX -- photon i draws from its own Philox4x32-10 stream
- *     keyed by (seed, i); draw k is word k%4 of block k/4.  This is the
- *     stream definition the HIP kernels use, so their FP64 path can be
- *     compared with this file photon-for-photon.
+ *     keyed by (seed, i): sequential words during creation, then one block
+ *     per element step (PhiloxStream below).  This is the stream definition
+ *     the HIP kernels use, so their FP64 path can be compared with this
+ *     file photon-for-photon.
  *
  * Floating-point expressions that feed the tallies are written in the same
  * order as the reference so that the MT19937 mode reproduces its bits with
@@ -82,6 +83,7 @@ struct MtStream {
   std::mt19937 g;
   explicit MtStream(uint64_t s) : g((uint32_t)(s & 0xffffffffu)) {}
   void begin_photon(uint64_t) {}
+  void begin_step(uint32_t) {}
   uint32_t next() { return (uint32_t)g(); }
 };
 
@@ -104,18 +106,29 @@ void philox_block(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t o
 }
 
 /* One Philox stream per photon: key = (seed lo, seed hi), counter =
-   (block, tag, photon lo, photon hi); draw k is word k%4 of block k/4. */
+   (block, tag, photon lo, photon hi).  Photon creation and the first free
+   path read words 0, 1, 2, ... of blocks 0, 1, ... in order; element step s
+   of the walk (one WhichFace and everything up to the next) reads words
+   0..2 of block 0x80000000 | s.  This is the HIP kernels' definition. */
 struct PhiloxStream {
   uint32_t key[2];
   uint32_t tag;
   uint64_t pid = 0;
   uint32_t k = 0;
+  bool stepping = false;
   uint32_t buf[4];
   explicit PhiloxStream(uint64_t seed, uint32_t tag_ = 0) : tag(tag_) {
     key[0] = (uint32_t)seed; key[1] = (uint32_t)(seed >> 32);
   }
-  void begin_photon(uint64_t id) { pid = id; k = 0; }
+  void begin_photon(uint64_t id) { pid = id; k = 0; stepping = false; }
+  void begin_step(uint32_t s) {
+    uint32_t ctr[4] = {0x80000000u | s, tag, (uint32_t)pid, (uint32_t)(pid >> 32)};
+    philox_block(ctr, key, buf);
+    stepping = true;
+    k = 0;
+  }
   uint32_t next() {
+    if (stepping) return buf[k++];
     if ((k & 3u) == 0) {
       uint32_t ctr[4] = {k >> 2, tag, (uint32_t)pid, (uint32_t)(pid >> 32)};
       philox_block(ctr, key, buf);
@@ -784,9 +797,11 @@ struct Walker {
   void propagate(Packet &ph) {
     const bool d3 = m.dim == 3;
     double prop, dist = 0, ds;
+    uint32_t step = 0;
     for (;;) {
       prop = -log(open()) / m.mus[ph.curel];
       for (;;) {
+        rng.begin_step(step++);
         int wf = d3 ? which_face3(ph, &dist) : which_face2(ph, &dist);
         if (wf == -1) { ++loss; led[4] += ph.weight; return; }
         ds = fmin(prop, dist);

@@ -22,9 +22,10 @@
 #include <vector>
 
 namespace vmc {
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, int *blocks_per_cu);
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, hipStream_t s,
-                       const WalkParams &P);
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, size_t lds_bytes,
+                          int *blocks_per_cu);
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, size_t lds_bytes,
+                       hipStream_t s, const WalkParams &P);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, double inv_scale,
                          hipStream_t s);
@@ -739,10 +740,21 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
   const bool fd = omega > 0.0;
   double phase0 = run->phase0;
   if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
-  int block = run->block_size > 0 ? run->block_size : 256;
-  if (block % 64 != 0 || block > 256) return VMC_ERR_INVALID_ARGUMENT;
-  int nrep = run->replicas > 0 ? run->replicas : 8;
-  if (nrep > 64) return VMC_ERR_INVALID_ARGUMENT;
+  // defaults from a sweep on the 384k-tet box (block x LDS slots): FP32
+  // transport runs best as one 1024-lane workgroup per CU sharing one large
+  // tally cache, FP64 (occupancy 2 waves/SIMD) as 256-lane workgroups
+  int block = run->block_size > 0 ? run->block_size : (f64 ? 256 : 1024);
+  if (block % 64 != 0 || block > (f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
+  // replicas == 0: one per XCD, added with L2 atomics; > 0: that many,
+  // picked by workgroup index and added with device-scope atomics
+  const bool xcc = run->replicas <= 0;
+  int nrep = xcc ? 8 : run->replicas;
+  if (nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
+  // LDS tally cache: 0 = default size, -1 = off, else a power of two
+  const int def_slots = f64 ? (fd ? 2048 : 4096) : (fd ? 4096 : 8192);
+  int slots = run->lds_slots == 0 ? def_slots : (run->lds_slots < 0 ? 0 : run->lds_slots);
+  if (slots > 16384 || (slots & (slots - 1)) != 0) return VMC_ERR_INVALID_ARGUMENT;
+  const size_t lds_bytes = (size_t)slots * (4 + 8 * (fd ? 2 : 1));
   const int64_t N = run->Nphoton;
   const size_t ne_slots = (size_t)m->Ne * (fd ? 2 : 1), nb_slots = (size_t)m->Nb * 4;
   int aborted = 0;
@@ -755,9 +767,12 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
     hip_check(hipSetDevice(m->device), "hipSetDevice");
     upload_params(*m, f64);
     int bpc = 0;
-    hip_check(walk_occupancy(m->dim, f64, fd, fixed, block, &bpc), "occupancy");
-    if (run->blocks_per_cu > 0) bpc = std::min(bpc > 0 ? bpc : 1, run->blocks_per_cu);
-    if (bpc < 1) bpc = 1;
+    hip_check(walk_occupancy(m->dim, f64, fd, fixed, block, lds_bytes, &bpc), "occupancy");
+    if (bpc < 1) {
+      g_last_error = "workgroup does not fit on a CU (block size / LDS tally cache too large)";
+      return VMC_ERR_INVALID_ARGUMENT;
+    }
+    if (run->blocks_per_cu > 0) bpc = std::min(bpc, run->blocks_per_cu);
     grid = bpc * m->num_cus;
     m->etally.ensure(ne_slots * nrep);
     m->btally.ensure(nb_slots * nrep);
@@ -787,6 +802,8 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
     P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
     P.refill_min = std::min(std::max(run->refill_min, 1), 64);
     P.chunk = 128;
+    P.xcc_replicas = xcc ? 1 : 0;
+    P.lds_slots = slots;
     P.phase0 = phase0;
     P.cph0 = cos(phase0);
     P.sph0 = sin(phase0);
@@ -804,7 +821,7 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
       hip_check(launch_fill_u64(m->counters.p, 1, (unsigned long long)done, m->stream), "fill");
       P.n_end = (uint64_t)end;
       hip_check(hipEventRecord(m->ev0, m->stream), "event");
-      hip_check(launch_walk(m->dim, f64, fd, fixed, grid, block, m->stream, P), "walk");
+      hip_check(launch_walk(m->dim, f64, fd, fixed, grid, block, lds_bytes, m->stream, P), "walk");
       hip_check(hipEventRecord(m->ev1, m->stream), "event");
       hip_check(hipEventSynchronize(m->ev1), "sync");
       float ms = 0.f;

@@ -39,13 +39,13 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    const uint64_t p0 = (uint64_t)c0 * 0xD2511F53u;   // one v_mad_u64_u32 each
+    const uint64_t p1 = (uint64_t)c2 * 0xCD9E8D57u;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
     c0 = n0;
-    c1 = lo1;
+    c1 = (uint32_t)p1;
     c2 = n2;
-    c3 = lo0;
+    c3 = (uint32_t)p0;
   }
   o0 = c0;
   o1 = c1;
@@ -53,8 +53,16 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   o3 = c3;
 }
 
-// Draw k of photon p is word k%4 of block (k/4, tag, p_lo, p_hi).
-struct Stream {
+// Stream of photon p (64-bit id) under key (seed) and tag:
+//  * photon creation and its first free path read words 0, 1, 2, ... of
+//    blocks (0, tag, p), (1, tag, p), ... in order (SeqDraws);
+//  * element step s of the walk reads words 0..2 of block
+//    (0x80000000 | s, tag, p) (StepDraws).  A step needs at most three
+//    draws (scattering: angle, azimuth, free path; crossing: fresh free
+//    path, roulette, Fresnel), and every lane of a wave generates its step
+//    block at the same converged point, so the generator never runs inside
+//    a divergent branch of the walk.
+struct SeqDraws {
   uint32_t plo, phi, k, b1, b2, b3;
   __device__ __forceinline__ uint32_t next(const WalkParams &P) {
     uint32_t r;
@@ -66,6 +74,20 @@ struct Stream {
       b2 = b3;
     }
     ++k;
+    return r;
+  }
+};
+
+struct StepDraws {
+  uint32_t w0, w1, w2, j;
+  __device__ __forceinline__ void begin(uint32_t plo, uint32_t phi, uint32_t step, const WalkParams &P) {
+    uint32_t w3;
+    philox4x32_10(0x80000000u | step, P.tag, plo, phi, P.key0, P.key1, w0, w1, w2, w3);
+    j = 0;
+  }
+  __device__ __forceinline__ uint32_t next(const WalkParams &) {
+    const uint32_t r = j == 0u ? w0 : (j == 1u ? w1 : w2);
+    ++j;
     return r;
   }
 };
@@ -116,14 +138,64 @@ template <> struct Tol<double> { static constexpr double out = 1e-12; };
 // Tally adds
 // ---------------------------------------------------------------------------
 
-template <bool FIXED> __device__ __forceinline__ void tally_add(void *base, int64_t idx, double v, double scale);
-template <> __device__ __forceinline__ void tally_add<false>(void *base, int64_t idx, double v, double) {
-  unsafeAtomicAdd(reinterpret_cast<double *>(base) + idx, v);
+// With one replica per XCD (`local`), every add to a replica comes from
+// the CUs behind one L2, so a workgroup-scope atomic -- performed in that
+// L2 -- is exact; device-scope atomics would be forwarded past the
+// XCD-private L2 to memory.  Kernel completion writes the L2s back before
+// the reduction kernel reads the replicas.
+template <bool FIXED>
+__device__ __forceinline__ void tally_add(void *base, int64_t idx, double v, double scale, bool local) {
+  if constexpr (FIXED) {
+    unsigned long long *a = reinterpret_cast<unsigned long long *>(base) + idx;
+    const unsigned long long q = (unsigned long long)__double2ll_rn(v * scale);
+    if (local) __hip_atomic_fetch_add(a, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_fetch_add(a, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    double *a = reinterpret_cast<double *>(base) + idx;
+    if (local) __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
-template <> __device__ __forceinline__ void tally_add<true>(void *base, int64_t idx, double v, double scale) {
-  const long long q = __double2ll_rn(v * scale);
-  atomicAdd(reinterpret_cast<unsigned long long *>(base) + idx, (unsigned long long)q);
-}
+
+// Per-workgroup LDS tally cache.  A slot is claimed by the first element
+// that hashes to it and keeps that element for the rest of the launch; adds
+// to a claimed element stay in LDS and are flushed to the global replica
+// once, when the workgroup finishes.  Every photon starts at a light source,
+// so the slots fill with the elements next to the sources, which are the
+// ones that take the most adds and would otherwise contend on one address.
+template <bool FIXED, int W> struct LdsTally {
+  int *tag;      // [S] element id or -1
+  void *val;     // [S * W] doubles or fixed-point int64
+  int mask;      // S - 1, or -1 when the cache is off
+  __device__ __forceinline__ int slot_of(int el) const {
+    return (int)(((uint32_t)el * 0x9E3779B1u) >> 8) & mask;
+  }
+  // true when the add went into LDS
+  __device__ __forceinline__ bool add(int el, double v0, double v1, double scale) {
+    if (mask < 0) return false;
+    const int sl = slot_of(el);
+    int t = __hip_atomic_load(&tag[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (t == -1) {
+      const int old = atomicCAS(&tag[sl], -1, el);
+      t = old == -1 ? el : old;
+    }
+    if (t != el) return false;
+    if constexpr (FIXED) {
+      unsigned long long *a = reinterpret_cast<unsigned long long *>(val) + (size_t)sl * W;
+      atomicAdd(a, (unsigned long long)__double2ll_rn(v0 * scale));
+      if (W == 2) atomicAdd(a + 1, (unsigned long long)__double2ll_rn(v1 * scale));
+    } else {
+      double *a = reinterpret_cast<double *>(val) + (size_t)sl * W;
+      atomicAdd(a, v0);
+      if (W == 2) atomicAdd(a + 1, v1);
+    }
+    return true;
+  }
+};
+
+// Index of the XCD (accelerator complex die) running this wave:
+// hardware register XCC_ID (id 20, bits 3:0) on gfx940 and later.
+__device__ __forceinline__ int xcc_id() { return (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15u); }
 
 // ---------------------------------------------------------------------------
 // Photon state and helpers
@@ -133,6 +205,7 @@ template <class R> struct Photon {
   R x[3], d[3];
   R w, prop;
   R cph, sph;   // cos/sin of the phase (frequency domain)
+  R acc0, acc1; // deposits of the current element visit (ER, EI), flushed on leaving
   int el, face, nsteps;
 };
 
@@ -153,9 +226,9 @@ __device__ __forceinline__ void load_elem(const Elem<NF, R> *__restrict__ E, int
 // the unpolarised Fresnel formula in cosine form, which equals the
 // reference's sin/tan form given Snell's law, including its normal-incidence
 // limit.  Returns true when the packet is reflected.
-template <int D, class R>
+template <int D, class R, class S>
 __device__ __forceinline__ bool fresnel(Photon<R> &p, const R n[3], R n1, R n2, bool refract,
-                                        Stream &rs, const WalkParams &P) {
+                                        S &rs, const WalkParams &P) {
   const R eta = n1 / n2;
   R ci = p.d[0] * n[0] + p.d[1] * n[1];
   if (D == 3) ci += p.d[2] * n[2];
@@ -186,8 +259,8 @@ __device__ __forceinline__ bool fresnel(Photon<R> &p, const R n[3], R n1, R n2, 
 }
 
 // Henyey-Greenstein scattering, MC3D::ScatterPhoton (MC3D.hpp:1297-1340).
-template <class R>
-__device__ __forceinline__ void scatter3(Photon<R> &p, R g, Stream &rs, const WalkParams &P) {
+template <class R, class S>
+__device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkParams &P) {
   R cost;
   if (g != R(0)) {
     const R xi = Unif<R>::closed(rs.next(P));
@@ -226,8 +299,8 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, Stream &rs, const Wa
 
 // 2D Henyey-Greenstein, MC2D::ScatterPhoton (MC2D.hpp:1135-1155):
 // theta = 2 atan(tan(pi xi) (1-g)/(1+g)), applied as a rotation.
-template <class R>
-__device__ __forceinline__ void scatter2(Photon<R> &p, R g, Stream &rs, const WalkParams &P) {
+template <class R, class S>
+__device__ __forceinline__ void scatter2(Photon<R> &p, R g, S &rs, const WalkParams &P) {
   const R xi = Unif<R>::halfup(rs.next(P));
   R s, c;
   if (g != R(0)) {
@@ -267,7 +340,7 @@ template <class R> __device__ __forceinline__ R gauss_angle(R xi, R sig, R h) {
 // CreatePhoton (MC3D.hpp:1060-1294, MC2D.hpp:958-1132).  Returns the source's
 // mother element and initial face.
 template <int D, class R>
-__device__ __forceinline__ void create_photon(Photon<R> &p, Stream &rs, const WalkParams &P) {
+__device__ __forceinline__ void create_photon(Photon<R> &p, SeqDraws &rs, const WalkParams &P) {
   // source entry: first CDF value above xi, clamped (MC2D.hpp:966-969)
   const double xi = Unif<double>::closed(rs.next(P));
   int lo = 0, hi = P.nsrc;
@@ -359,6 +432,8 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, Stream &rs, const Wa
     }
   }
   p.w = R(1);
+  p.acc0 = R(0);
+  p.acc1 = R(0);
   p.cph = R(P.cph0);
   p.sph = R(P.sph0);
   p.nsteps = 0;
@@ -368,21 +443,49 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, Stream &rs, const Wa
 // The walk
 // ---------------------------------------------------------------------------
 
+// FP32 kernels fit 16 waves per CU in 128 VGPRs, so they take workgroups of
+// up to 1024 lanes (one large LDS tally cache per CU); FP64 kernels need more
+// registers per lane and stay at 256.
+template <class R> struct MaxBlock { static constexpr int value = std::is_same<R, float>::value ? 1024 : 256; };
+
 template <int D, class R, bool FD, bool FIXED>
-__global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
+__global__ void __launch_bounds__(MaxBlock<R>::value) walk_kernel(const WalkParams P) {
   constexpr int NF = D + 1;
   typedef typename Vec4T<R>::type V4;
   const Elem<NF, R> *__restrict__ E = reinterpret_cast<const Elem<NF, R> *>(P.elems);
-  const int rep = (int)(blockIdx.x % (unsigned)P.nrep);
+  const bool local = P.xcc_replicas != 0;
+  const int rep = local ? (xcc_id() % P.nrep) : (int)(blockIdx.x % (unsigned)P.nrep);
   void *const etally = reinterpret_cast<char *>(P.etally) + (size_t)rep * P.ne * (FD ? 2 : 1) * 8;
   void *const btally = reinterpret_cast<char *>(P.btally) + (size_t)rep * P.nb * 4 * 8;
   const int lane = (int)__lane_id();
 
   Photon<R> p;
-  Stream rs;
+  uint32_t plo = 0, phi = 0;   // photon id: the key of its random stream
   bool alive = false;
-  double l_abs = 0, l_exit = 0, l_kill = 0, l_gain = 0, l_lost = 0;
-  uint32_t steps = 0, losses = 0;
+  // absorbed weight accumulates per lane; the rare end-of-photon sums go to
+  // per-wave LDS slots [exited, killed, gained, lost, loss count]
+  double l_abs = 0;
+  __shared__ double s_led[MaxBlock<R>::value / 64][5];
+  const int wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) s_led[wid][q] = 0.0;
+  }
+  constexpr int W = FD ? 2 : 1;
+  extern __shared__ __align__(16) unsigned char s_dyn[];
+  LdsTally<FIXED, W> cache{reinterpret_cast<int *>(s_dyn),
+                           s_dyn + (size_t)(P.lds_slots > 0 ? P.lds_slots : 0) * 4,
+                           P.lds_slots > 0 ? P.lds_slots - 1 : -1};
+  for (int i = threadIdx.x; i < P.lds_slots; i += blockDim.x) {
+    cache.tag[i] = -1;
+    if constexpr (FIXED) {
+      for (int w = 0; w < W; ++w) reinterpret_cast<unsigned long long *>(cache.val)[(size_t)i * W + w] = 0ull;
+    } else {
+      for (int w = 0; w < W; ++w) reinterpret_cast<double *>(cache.val)[(size_t)i * W + w] = 0.0;
+    }
+  }
+  __syncthreads();
+  uint64_t steps = 0;   // wave-uniform
 
   // wave-private photon id pool [pool_next, pool_end)
   uint64_t pool_next = 0, pool_end = 0;
@@ -416,16 +519,20 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
         const bool invalid = !alive && pid >= P.n_end;
         if (__ballot(invalid)) exhausted = true;
         if (!alive && !invalid) {
-          rs.plo = (uint32_t)pid;
-          rs.phi = (uint32_t)(pid >> 32);
-          rs.k = 0;
-          create_photon<D, R>(p, rs, P);
-          p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
+          plo = (uint32_t)pid;
+          phi = (uint32_t)(pid >> 32);
+          SeqDraws cs{plo, phi, 0u, 0u, 0u, 0u};
+          create_photon<D, R>(p, cs, P);
+          p.prop = -m_log(Unif<R>::open(cs.next(P))) / E[p.el].opt.y;
           alive = true;
         }
       }
     }
-    if (__ballot(alive) == 0) break;
+    const uint64_t live = __ballot(alive);
+    if (live == 0) break;
+    steps += __popcll(live);
+    StepDraws rs;
+    rs.begin(plo, phi, (uint32_t)p.nsteps, P);   // converged: all lanes at once
     if (!alive) continue;
 
     // ---- one element step ----
@@ -436,8 +543,22 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
     V4 opt;
     i4 nb;
     load_elem<NF, R>(E, p.el, pl, opt, nb);
-    ++steps;
     ++p.nsteps;
+    // one element visit (entry to exit, scatterings inside included) adds
+    // its deposits once: to the workgroup's LDS cache, else to the replica
+    auto flush_visit = [&]() {
+      if constexpr (!FD) {
+        if (p.acc0 != R(0) && !cache.add(p.el, (double)p.acc0, 0.0, P.fix_scale))
+          tally_add<FIXED>(etally, p.el, (double)p.acc0, P.fix_scale, local);
+      } else {
+        if (!cache.add(p.el, (double)p.acc0, (double)p.acc1, P.fix_scale)) {
+          tally_add<FIXED>(etally, 2 * (int64_t)p.el, (double)p.acc0, P.fix_scale, local);
+          tally_add<FIXED>(etally, 2 * (int64_t)p.el + 1, (double)p.acc1, P.fix_scale, local);
+        }
+      }
+      p.acc0 = R(0);
+      p.acc1 = R(0);
+    };
     // exit face: smallest positive plane distance over faces moving outward,
     // the current face excluded (MC3D.hpp:1020-1056)
     R bn = R(1), bc = R(0);
@@ -467,8 +588,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
     }
     if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
       // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
-      l_lost += (double)p.w;
-      ++losses;
+      flush_visit();
+      atomicAdd(&s_led[wid][3], (double)p.w);
+      atomicAdd(&s_led[wid][4], 1.0);
       alive = false;
       continue;
     }
@@ -488,7 +610,7 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
     const R dabs = kF64 ? p.w - p.w * att : -p.w * em;
     if constexpr (!FD) {
       const R dep = mua > R(0) ? (kF64 ? (R(1) - att) * p.w : dabs) : p.w * ds;
-      if (dep != R(0)) tally_add<FIXED>(etally, p.el, (double)dep, P.fix_scale);
+      p.acc0 += dep;
     } else {
       // w e^{-i phi} (1 - e^{-(mua + i k) ds}), MC3D.hpp:1519-1522, with the
       // phase advance b = k ds taken from half-angle terms
@@ -506,8 +628,8 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
         er = p.w * (p.cph * qq + att * p.sph * sb);
         ei = p.w * (-p.sph * qq + att * p.cph * sb);
       }
-      tally_add<FIXED>(etally, 2 * (int64_t)p.el, (double)er, P.fix_scale);
-      tally_add<FIXED>(etally, 2 * (int64_t)p.el + 1, (double)ei, P.fix_scale);
+      p.acc0 += er;
+      p.acc1 += ei;
       if constexpr (kF64) {
         p.cph = c1;
         p.sph = s1;
@@ -533,8 +655,9 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
 
     if (code < 0) {
       if (code == kNbNone) {
-        l_lost += (double)p.w;
-        ++losses;
+        flush_visit();
+        atomicAdd(&s_led[wid][3], (double)p.w);
+        atomicAdd(&s_led[wid][4], 1.0);
         alive = false;
         continue;
       }
@@ -555,22 +678,23 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
       }
       const size_t bo = (size_t)ib * 4;
       if (!FD) {
-        tally_add<FIXED>(btally, bo, (double)p.w, P.fix_scale);
+        tally_add<FIXED>(btally, bo, (double)p.w, P.fix_scale, local);
       } else {
-        tally_add<FIXED>(btally, bo, (double)(p.w * p.cph), P.fix_scale);
-        tally_add<FIXED>(btally, bo + 1, (double)(-p.w * p.sph), P.fix_scale);
+        tally_add<FIXED>(btally, bo, (double)(p.w * p.cph), P.fix_scale, local);
+        tally_add<FIXED>(btally, bo + 1, (double)(-p.w * p.sph), P.fix_scale, local);
       }
       if constexpr (D == 2) {
         // exitance weighted by 1/(dir . outward normal), MC2D.hpp:1410-1437
         const R dn = p.d[0] * nrm[0] + p.d[1] * nrm[1];
         if (!FD) {
-          tally_add<FIXED>(btally, bo + 2, (double)(p.w / dn), P.fix_scale);
+          tally_add<FIXED>(btally, bo + 2, (double)(p.w / dn), P.fix_scale, local);
         } else {
-          tally_add<FIXED>(btally, bo + 2, (double)(p.w * p.cph / dn), P.fix_scale);
-          tally_add<FIXED>(btally, bo + 3, (double)(-p.w * p.sph / dn), P.fix_scale);
+          tally_add<FIXED>(btally, bo + 2, (double)(p.w * p.cph / dn), P.fix_scale, local);
+          tally_add<FIXED>(btally, bo + 3, (double)(-p.w * p.sph / dn), P.fix_scale, local);
         }
       }
-      l_exit += (double)p.w;
+      flush_visit();
+      atomicAdd(&s_led[wid][0], (double)p.w);
       alive = false;
       continue;
     }
@@ -590,12 +714,13 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
     if (p.w < R(P.weight0)) {
       // Russian roulette at the crossing (MC3D.hpp:1597-1604)
       if (Unif<R>::closed(rs.next(P)) > R(P.chance)) {
-        l_kill += (double)p.w;
+        flush_visit();
+        atomicAdd(&s_led[wid][1], (double)p.w);
         alive = false;
         continue;
       }
       const R wn = p.w / R(P.chance);
-      l_gain += (double)wn - (double)p.w;
+      atomicAdd(&s_led[wid][2], (double)wn - (double)p.w);
       p.w = wn;
     }
     if (n_c != n_n) reflected = fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P);
@@ -606,34 +731,43 @@ __global__ void __launch_bounds__(256) walk_kernel(const WalkParams P) {
     if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= mus_c / mus_n;
     if (!back) {
       // the neighbour does not list this element: back-lookup failure
-      l_lost += (double)p.w;
-      ++losses;
+      flush_visit();
+      atomicAdd(&s_led[wid][3], (double)p.w);
+      atomicAdd(&s_led[wid][4], 1.0);
       alive = false;
       continue;
     }
+    flush_visit();
     p.el = next;
     p.face = entry;
   }
 
-  // ---- flush per-lane counters ----
+  // ---- flush the counters: one set of global atomics per wave ----
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    l_abs += __shfl_xor(l_abs, off);
-    l_exit += __shfl_xor(l_exit, off);
-    l_kill += __shfl_xor(l_kill, off);
-    l_gain += __shfl_xor(l_gain, off);
-    l_lost += __shfl_xor(l_lost, off);
-    steps += __shfl_xor(steps, off);
-    losses += __shfl_xor(losses, off);
+  for (int off = 32; off > 0; off >>= 1) l_abs += __shfl_xor(l_abs, off);
+  __syncthreads();
+  // ---- flush the LDS tally cache into this workgroup's replica ----
+  for (int i = threadIdx.x; i < P.lds_slots; i += blockDim.x) {
+    const int el = cache.tag[i];
+    if (el < 0) continue;
+    for (int w = 0; w < W; ++w) {
+      if constexpr (FIXED) {
+        const unsigned long long q = reinterpret_cast<unsigned long long *>(cache.val)[(size_t)i * W + w];
+        if (q) atomicAdd(reinterpret_cast<unsigned long long *>(etally) + (size_t)el * W + w, q);
+      } else {
+        const double v = reinterpret_cast<double *>(cache.val)[(size_t)i * W + w];
+        if (v != 0.0) unsafeAtomicAdd(reinterpret_cast<double *>(etally) + (size_t)el * W + w, v);
+      }
+    }
   }
   if (lane == 0) {
     unsafeAtomicAdd(&P.ledger[0], l_abs);
-    unsafeAtomicAdd(&P.ledger[1], l_exit);
-    unsafeAtomicAdd(&P.ledger[2], l_kill);
-    unsafeAtomicAdd(&P.ledger[3], l_gain);
-    unsafeAtomicAdd(&P.ledger[4], l_lost);
+    unsafeAtomicAdd(&P.ledger[1], s_led[wid][0]);
+    unsafeAtomicAdd(&P.ledger[2], s_led[wid][1]);
+    unsafeAtomicAdd(&P.ledger[3], s_led[wid][2]);
+    unsafeAtomicAdd(&P.ledger[4], s_led[wid][3]);
     atomicAdd(&P.counters[1], (unsigned long long)steps);
-    atomicAdd(&P.counters[2], (unsigned long long)losses);
+    atomicAdd(&P.counters[2], (unsigned long long)s_led[wid][4]);
   }
 }
 
@@ -686,13 +820,15 @@ static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed) {
 #undef VMC_PICK
 }
 
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed), block, 0);
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, size_t lds_bytes,
+                          int *blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed), block,
+                                                      lds_bytes);
 }
 
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, hipStream_t s,
-                       const WalkParams &P) {
-  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed), dim3(grid), dim3(block), 0, s, P);
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, size_t lds_bytes,
+                       hipStream_t s, const WalkParams &P) {
+  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed), dim3(grid), dim3(block), lds_bytes, s, P);
   return hipGetLastError();
 }
 

@@ -90,6 +90,8 @@ struct WalkParams {
   uint32_t step_cap;
   int32_t refill_min;
   int32_t chunk;
+  int32_t xcc_replicas;         // 1: replica = XCD id, workgroup-scope (L2) atomics
+  int32_t lds_slots;            // LDS tally cache entries per workgroup (0 = none)
   double phase0, cph0, sph0;    // initial phase and its cos/sin
   double kfac;                  // omega / c0: k = kfac * n
   double weight0, chance;
