@@ -50,11 +50,55 @@ def parse():
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
+    ap.add_argument("--copy-bytes", type=float, default=0,
+                    help="only run a streaming copy kernel (out = -in, float32) over this many bytes "
+                         "three times: the reference dispatch that scales FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+def copy_reference(nbytes: int):
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch.neg float32", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "launches": 3}), flush=True)
+
+
+def traffic_per_step():
+    """HBM bytes per element step of the FP32 CW walk, from the committed
+    FETCH_SIZE / WRITE_SIZE profile of this workload (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "walk3d_fp32_box384k_hbm_traffic.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("hbm_bytes_per_element_step:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def cpu_baseline(p, target_s: float):
     """Time the reference engine (or, without it, the oracle) on the host."""
+    # the OpenMP runtime reads these when the reference library first loads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
@@ -69,7 +113,8 @@ def cpu_baseline(p, target_s: float):
     r = run(p, threads)
     return {"value": n / r.seconds, "unit": "photons/s", "cores": threads, "kind": kind,
             "sample": f"{n} photons of the same 384k-tet problem, MonteCarlo() only, "
-                      f"{threads} OpenMP threads, {r.seconds:.1f} s"}
+                      f"{threads} OpenMP threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                      f"OMP_PLACES={os.environ.get('OMP_PLACES')}) on {cpu_model()}, {r.seconds:.1f} s"}
 
 
 def main():
@@ -78,6 +123,9 @@ def main():
         print(f"bench.py runs one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is a "
               "design note, not a measured configuration", file=sys.stderr)
         sys.exit(2)
+    if a.copy_bytes:
+        copy_reference(int(a.copy_bytes))
+        return
     import valomc_amd as vmc
     from valomc_amd import meshgen
 
@@ -99,8 +147,10 @@ def main():
     kernel_s = sum(r.kernel_ms for r in runs) / 1e3
     steps_total = sum(r.steps for r in runs)
     achieved = steps_total * B_STEP[("fp32", False)] / kernel_s / 1e9
+    tps = traffic_per_step()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if tps is None else round(tps * steps_total / a.steps),
                 "kernel": "walk_kernel<3,float,CW,fp64-tally>",
                 "bytes_per_step": B_STEP[("fp32", False)],
                 "steps_per_launch": steps_total // a.steps,

@@ -82,7 +82,9 @@ typedef struct vmc_run_desc {
   int32_t lds_slots;     /* per-workgroup LDS tally cache entries (power of 2);
                             0 = default (FP32 8192 CW / 4096 FD, FP64 4096 CW
                             / 2048 FD), -1 = off, at most 16384                */
-  int32_t reserved[4];
+  int32_t queue_len;     /* per-wave LDS queue of tally adds drained in batches
+                            (multiple of 64); 0 = default (256), -1 = off      */
+  int32_t reserved[3];
 } vmc_run_desc;
 
 typedef struct vmc_output {

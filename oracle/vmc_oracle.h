@@ -58,7 +58,7 @@ typedef struct oracle_result {
 /* rng_mode */
 #define ORACLE_RNG_MT19937 0 /* reference streams: thread t uses mt19937(seed+t),
                                 photons split N/T, remainder to the last thread */
-#define ORACLE_RNG_PHILOX 1  /* one Philox4x32-10 stream per photon id          */
+#define ORACLE_RNG_PHILOX 1  /* per-photon Philox4x32-10 + xoshiro128** streams */
 
 /* Return 0 on success, 1 + mcerror code on an input error
    (cpp/Errors.hpp:4-21 order). */

@@ -115,6 +115,35 @@ def test_fixed_point_tallies_are_bit_reproducible(vmc):
     assert rel_l2(c.ER, a.ER) < 1e-8
 
 
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_iso_gauss_fd",
+                                  "p2_mirror_hetero"])
+def test_tally_paths_agree_bit_for_bit(vmc, name):
+    """Fixed-point adds are exact and order-independent, so the LDS cache,
+    the per-wave LDS queue and direct atomics must give the same bits, for
+    element words (CW and FD) and every boundary word (3D and 2D)."""
+    p = _golden(name)
+    p.Nphoton = 50000
+    print(f"case {name}", flush=True)
+    with vmc.Problem(p) as prob:
+        run = lambda **kw: prob.run(p.Nphoton, seed=11, f=p.f, phase0=p.phase0, precision=vmc.FP32,
+                                    tally=vmc.TALLY_FIXED64, **kw)
+        base = run()
+        for kw in (dict(lds_slots=-1, queue_len=-1), dict(lds_slots=-1), dict(queue_len=-1),
+                   dict(lds_slots=7, queue_len=1), dict(block_size=64, queue_len=1024)):
+            r = run(**kw)
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(base, k)), (kw, k)
+
+
+def test_tally_queue_rejects_bad_lengths(vmc):
+    """More than 1024 queued sectors per wave is refused."""
+    p = _golden("p3_pencil_sphere")
+    with vmc.Problem(p) as prob:
+        for q in (1025, 8192):
+            with pytest.raises(vmc.VmcError):
+                prob.run(1000, queue_len=q)
+
+
 def test_results_do_not_depend_on_launch_shape(vmc):
     p = _golden("p2_square_cosinic")
     p.Nphoton = 50000

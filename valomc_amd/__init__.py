@@ -53,7 +53,8 @@ class RunDesc(C.Structure):
         ("f", C.c_double), ("phase0", C.c_double), ("Nphoton", C.c_int64), ("seed", C.c_uint64),
         ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
-        ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("queue_len", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -150,6 +151,7 @@ class RunResult:
     grid_blocks: int
     block_size: int
     status: int = 0
+    rechecks: int = 0
 
 
 class Problem:
@@ -219,13 +221,14 @@ class Problem:
     def run(self, nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-            lds_slots: int = 0,
+            lds_slots: int = 0, queue_len: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         rd = RunDesc()
         rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
         rd.precision, rd.tally = int(precision), int(tally)
         rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
         rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
+        rd.queue_len = int(queue_len)
         out = dict(ER=np.zeros(self.Ne), EI=np.zeros(self.Ne), EBR=np.zeros(self.Nb),
                    EBI=np.zeros(self.Nb), DEBR=np.zeros(self.Nb), DEBI=np.zeros(self.Nb))
         o = Output(*[out[k].ctypes.data for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")])
@@ -240,7 +243,7 @@ class Problem:
         return RunResult(status=st, loss=int(s.loss), steps=int(s.steps), kernel_ms=float(s.kernel_ms),
                          total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)),
                          photons=int(s.photons), grid_blocks=int(s.grid_blocks),
-                         block_size=int(s.block_size), **out)
+                         block_size=int(s.block_size), rechecks=int(s.rechecks), **out)
 
 
 class _EngineBase:

@@ -91,9 +91,11 @@ struct vmc_problem {
   std::vector<double> lsCDF;
   std::vector<SrcRec> src;
   std::vector<BndRec> bnd;
-  std::vector<i4> nbcode;
+  std::vector<i4> nbcode;            // per element (caller's numbering), neighbours as device ids
+  std::vector<int32_t> perm, inv;    // device order: perm[device id] = element, inv[element] = device id
 
   int device = 0, num_cus = 0;
+  int64_t lds_per_cu = 65536;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<unsigned char> elems32, elems64;
@@ -352,6 +354,74 @@ void prepare_directions(vmc_problem &m) {
   }
 }
 
+// Device order of the elements: sorted along a Morton (Z-order) curve of
+// their centroids, so elements that share faces get nearby ids.  A photon's
+// successive element records and tally words then share cache lines and
+// pages; meshes numbered by construction (a grid generator emits each of the
+// six tetrahedra of a cube in a separate block) otherwise put face
+// neighbours far apart.  Results are mapped back to the caller's numbering.
+uint64_t spread_bits3(uint64_t v) {   // 21 bits -> every third bit
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffull;
+  v = (v | v << 16) & 0x1f0000ff0000ffull;
+  v = (v | v << 8) & 0x100f00f00f00f00full;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+uint64_t spread_bits2(uint64_t v) {   // 32 bits -> every second bit
+  v &= 0xffffffffull;
+  v = (v | v << 16) & 0x0000ffff0000ffffull;
+  v = (v | v << 8) & 0x00ff00ff00ff00ffull;
+  v = (v | v << 4) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | v << 2) & 0x3333333333333333ull;
+  v = (v | v << 1) & 0x5555555555555555ull;
+  return v;
+}
+
+void build_order(vmc_problem &m) {
+  const int bits = m.dim == 3 ? 21 : 32;
+  double lo[3] = {0, 0, 0}, maxspan = 0;
+  for (int d = 0; d < m.dim; ++d) {
+    double a = m.x(0, d), b = a;
+    for (int64_t i = 1; i < m.Nr; ++i) {
+      a = std::min(a, m.x(i, d));
+      b = std::max(b, m.x(i, d));
+    }
+    lo[d] = a;
+    maxspan = std::max(maxspan, b - a);
+  }
+  // Morton cells about the size of the hexahedron six tetrahedra (or the
+  // quadrilateral two triangles) of a structured mesh fill, so the elements
+  // of one cell stay consecutive
+  double mean = 0;
+  for (int64_t e = 0; e < m.Ne; ++e) mean += std::fabs(m.dim == 3 ? volume3(m, e) : area2(m, e));
+  mean = m.Ne > 0 ? mean / (double)m.Ne : 0.0;
+  const double cell = m.dim == 3 ? std::cbrt(6.0 * mean) : std::sqrt(2.0 * mean);
+  const double maxcells = (double)((1ull << bits) - 1);
+  const double inv_cell = (cell > 0 && maxspan / cell < maxcells) ? 1.0 / cell
+                                                                  : (maxspan > 0 ? maxcells / maxspan : 1.0);
+  std::vector<uint64_t> key(m.Ne);
+  for (int64_t e = 0; e < m.Ne; ++e) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int d = 0; d < m.dim; ++d) {
+      double c = 0;
+      for (int k = 0; k < m.nv; ++k) c += m.x(m.h(e, k), d);
+      c /= m.nv;
+      q[d] = (uint64_t)std::min(std::max((c - lo[d]) * inv_cell, 0.0), maxcells);
+    }
+    key[e] = m.dim == 3 ? (spread_bits3(q[0]) | spread_bits3(q[1]) << 1 | spread_bits3(q[2]) << 2)
+                        : (spread_bits2(q[0]) | spread_bits2(q[1]) << 1);
+  }
+  m.perm.resize(m.Ne);
+  for (int64_t e = 0; e < m.Ne; ++e) m.perm[e] = (int32_t)e;
+  std::sort(m.perm.begin(), m.perm.end(), [&](int32_t a, int32_t b) {
+    return key[a] != key[b] ? key[a] < key[b] : a < b;
+  });
+  m.inv.resize(m.Ne);
+  for (int64_t d = 0; d < m.Ne; ++d) m.inv[m.perm[d]] = (int32_t)d;
+}
+
 // Per-source launch records (CreatePhoton, MC3D.hpp:1060-1294 / MC2D.hpp:958-1132)
 void build_source_records(vmc_problem &m) {
   m.src.assign(m.lsFace.size(), SrcRec{});
@@ -359,7 +429,7 @@ void build_source_records(vmc_problem &m) {
     SrcRec &s = m.src[i];
     const int64_t ib = m.lsFace[i], el = m.lsMother[i];
     s.ib = (int32_t)ib;
-    s.mother = (int32_t)el;
+    s.mother = m.inv[el];
     s.face = -1;
     for (int f = 0; f < m.nv; ++f)
       if (-1 - m.hn(el, f) == ib) { s.face = f; break; }
@@ -449,11 +519,12 @@ void build_source_records(vmc_problem &m) {
 template <int NF, class R>
 void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
   out.assign(m.Ne, Elem<NF, R>{});
-  for (int64_t e = 0; e < m.Ne; ++e) {
+  for (int64_t dev = 0; dev < m.Ne; ++dev) {
+    const int64_t e = m.perm[dev];
     double v[4][3] = {{0}};
     for (int c = 0; c < m.nv; ++c)
       for (int d = 0; d < m.dim; ++d) v[c][d] = m.x(m.h(e, c), d) - m.shift[d];
-    Elem<NF, R> &E = out[e];
+    Elem<NF, R> &E = out[dev];
     for (int f = 0; f < NF; ++f) {
       double nrm[3] = {0, 0, 0}, off = 0;
       if (m.dim == 3) {
@@ -489,6 +560,12 @@ void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
     E.opt.z = (R)m.n[e];
     E.opt.w = (R)m.g[e];
     E.nb = m.nbcode[e];
+    for (int f = 0; f < NF; ++f) {
+      const int code = m.nbcode[e][f];
+      const int64_t nx = code >= 0 ? (int64_t)m.perm[code >> 3] : -1;
+      E.nbo[f].x = nx >= 0 ? (R)m.mus[nx] : R(0);
+      E.nbo[f].y = nx >= 0 ? (R)m.n[nx] : R(0);
+    }
   }
 }
 
@@ -507,7 +584,7 @@ void build_nb_codes(vmc_problem &m) {
       int entry = 0, back = 0;
       for (int q = 0; q < m.nv; ++q)
         if (m.hn(v, q) == e) { entry = q; back = 1; break; }   // MC3D.hpp:1616-1628
-      code[f] = (int)((v << 3) | (back << 2) | entry);
+      code[f] = (int)(((int64_t)m.inv[v] << 3) | (back << 2) | entry);
     }
     m.nbcode[e] = i4{code[0], code[1], code[2], code[3]};
   }
@@ -643,6 +720,7 @@ int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
       }
       m->shift[dd] = 0.5 * (lo + hi);
     }
+    build_order(*m);
     build_source_records(*m);
     build_nb_codes(*m);
     build_boundary(*m);
@@ -691,6 +769,8 @@ int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
     hipDeviceProp_t prop;
     hip_check(hipGetDeviceProperties(&prop, m->device), "hipGetDeviceProperties");
     m->num_cus = prop.multiProcessorCount;
+    m->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
+                                                             : (int64_t)prop.sharedMemPerBlock;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&m->ev0), "hipEventCreate");
     hip_check(hipEventCreate(&m->ev1), "hipEventCreate");
@@ -740,9 +820,9 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
   const bool fd = omega > 0.0;
   double phase0 = run->phase0;
   if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
-  // defaults from a sweep on the 384k-tet box (block x LDS slots): FP32
-  // transport runs best as one 1024-lane workgroup per CU sharing one large
-  // tally cache, FP64 (occupancy 2 waves/SIMD) as 256-lane workgroups
+  // Launch shape: FP32 transport runs as one 1024-lane workgroup per CU (16
+  // waves sharing one sector cache), FP64 (occupancy 2 waves/SIMD) as
+  // 256-lane workgroups, two per CU.
   int block = run->block_size > 0 ? run->block_size : (f64 ? 256 : 1024);
   if (block % 64 != 0 || block > (f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
   // replicas == 0: one per XCD, added with L2 atomics; > 0: that many,
@@ -750,13 +830,27 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
   const bool xcc = run->replicas <= 0;
   int nrep = xcc ? 8 : run->replicas;
   if (nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
-  // LDS tally cache: 0 = default size, -1 = off, else a power of two
-  const int def_slots = f64 ? (fd ? 2048 : 4096) : (fd ? 4096 : 8192);
-  int slots = run->lds_slots == 0 ? def_slots : (run->lds_slots < 0 ? 0 : run->lds_slots);
-  if (slots > 16384 || (slots & (slots - 1)) != 0) return VMC_ERR_INVALID_ARGUMENT;
-  const size_t lds_bytes = (size_t)slots * (4 + 8 * (fd ? 2 : 1));
+  // run queue (sectors per wave): 0 = default, -1 = off (direct atomics)
+  const int W = fd ? 2 : 1, rbytes = f64 ? 8 : 4;
+  const int qlen = run->queue_len == 0 ? (fd ? 32 : 64) : (run->queue_len < 0 ? 0 : run->queue_len);
+  if (qlen > 1024) return VMC_ERR_INVALID_ARGUMENT;
+  // sector cache: 0 = fill the CU's LDS left over by the queues and lane
+  // buffers of the workgroups that fit by registers, -1 = off
+  int slots = run->lds_slots < 0 ? 0 : run->lds_slots;
+  if (run->lds_slots == 0) {
+    const int waves_per_cu = f64 ? 8 : 16;
+    const int bpc_target = std::max(1, waves_per_cu / (block / 64));
+    const size_t per_block = (size_t)m->lds_per_cu / bpc_target;
+    const size_t fixed_part = WalkLds(W, rbytes, 0, qlen, block).total + 1024;   // + ledger slots, alignment
+    const size_t entry = (size_t)kSectorElems * W * 8 + 4;
+    slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
+  }
+  if (slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
+  const size_t lds_bytes = WalkLds(W, rbytes, slots, qlen, block).total;
   const int64_t N = run->Nphoton;
-  const size_t ne_slots = (size_t)m->Ne * (fd ? 2 : 1), nb_slots = (size_t)m->Nb * 4;
+  // element tally words per replica: Ne rounded up to whole sectors
+  const int64_t ne_pad = (m->Ne + kSectorElems - 1) / kSectorElems * kSectorElems;
+  const size_t ne_slots = (size_t)ne_pad * W, nb_slots = (size_t)m->Nb * 4;
   int aborted = 0;
   float kernel_ms = 0.f;
   int grid = 0;
@@ -792,7 +886,7 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
     P.btally = m->btally.p;
     P.ledger = m->ledger.p;
     P.counters = m->counters.p;
-    P.ne = m->Ne;
+    P.ne = ne_pad;
     P.nb = m->Nb;
     P.nsrc = (int32_t)m->src.size();
     P.nrep = nrep;
@@ -804,6 +898,7 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
     P.chunk = 128;
     P.xcc_replicas = xcc ? 1 : 0;
     P.lds_slots = slots;
+    P.queue_len = qlen;
     P.phase0 = phase0;
     P.cph0 = cos(phase0);
     P.sph0 = sin(phase0);
@@ -847,8 +942,9 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
   // split raw tallies and normalise with the reference's expressions
   std::vector<double> ER(m->Ne), EI(m->Ne, 0.0), EBR(m->Nb), EBI(m->Nb), DEBR(m->Nb), DEBI(m->Nb), kn(m->Ne);
   for (int64_t i = 0; i < m->Ne; ++i) {
-    ER[i] = fd ? raw_e[2 * i] : raw_e[i];
-    if (fd) EI[i] = raw_e[2 * i + 1];
+    const int64_t dv = m->inv[i];
+    ER[i] = fd ? raw_e[2 * dv] : raw_e[dv];
+    if (fd) EI[i] = raw_e[2 * dv + 1];
     kn[i] = omega / kC0 * m->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
   }
   for (int64_t i = 0; i < m->Nb; ++i) {

@@ -5,8 +5,8 @@
 // finished takes the next photon ids from a wave-private pool that is filled
 // from one global counter in chunks, so the walk keeps all 64 lanes busy
 // until the photon budget runs out.  Photon i draws its random numbers from
-// its own Philox4x32-10 stream keyed by (seed, tag, i), so a result does not
-// depend on the launch shape or on which lane ran which photon.
+// its own stream keyed by (seed, tag, i), so a result does not depend on the
+// launch shape or on which lane ran which photon.
 //
 // Transport follows ValoMC's PropagatePhoton (cpp/3d/MC3D.hpp:1450-1638,
 // cpp/2d/MC2D.hpp:1307-1511) step for step and consumes random numbers in
@@ -55,13 +55,14 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 
 // Stream of photon p (64-bit id) under key (seed) and tag:
 //  * photon creation and its first free path read words 0, 1, 2, ... of
-//    blocks (0, tag, p), (1, tag, p), ... in order (SeqDraws);
-//  * element step s of the walk reads words 0..2 of block
-//    (0x80000000 | s, tag, p) (StepDraws).  A step needs at most three
-//    draws (scattering: angle, azimuth, free path; crossing: fresh free
-//    path, roulette, Fresnel), and every lane of a wave generates its step
-//    block at the same converged point, so the generator never runs inside
-//    a divergent branch of the walk.
+//    Philox blocks (0, tag, p), (1, tag, p), ... in order (SeqDraws);
+//  * the walk then draws from a xoshiro128** generator (Blackman & Vigna,
+//    "Scrambled linear pseudorandom number generators", 2018) whose 128-bit
+//    state is Philox block (0x80000000, tag, p).  A draw costs a few
+//    full-rate integer instructions and is made only in the branch that
+//    needs it, so lanes that cross a face without scattering pay nothing.
+//    Random 128-bit starting points in a 2^128 - 1 period make overlap of
+//    two photons' sequences negligible.
 struct SeqDraws {
   uint32_t plo, phi, k, b1, b2, b3;
   __device__ __forceinline__ uint32_t next(const WalkParams &P) {
@@ -78,16 +79,21 @@ struct SeqDraws {
   }
 };
 
-struct StepDraws {
-  uint32_t w0, w1, w2, j;
-  __device__ __forceinline__ void begin(uint32_t plo, uint32_t phi, uint32_t step, const WalkParams &P) {
-    uint32_t w3;
-    philox4x32_10(0x80000000u | step, P.tag, plo, phi, P.key0, P.key1, w0, w1, w2, w3);
-    j = 0;
+struct WalkRng {
+  uint32_t s0, s1, s2, s3;
+  __device__ __forceinline__ void seed(uint32_t plo, uint32_t phi, const WalkParams &P) {
+    philox4x32_10(0x80000000u, P.tag, plo, phi, P.key0, P.key1, s0, s1, s2, s3);
+    if ((s0 | s1 | s2 | s3) == 0u) s0 = 1u;   // the all-zero state is a fixed point
   }
   __device__ __forceinline__ uint32_t next(const WalkParams &) {
-    const uint32_t r = j == 0u ? w0 : (j == 1u ? w1 : w2);
-    ++j;
+    const uint32_t r = __builtin_rotateleft32(s1 * 5u, 7) * 9u;
+    const uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = __builtin_rotateleft32(s3, 11);
     return r;
   }
 };
@@ -138,56 +144,39 @@ template <> struct Tol<double> { static constexpr double out = 1e-12; };
 // Tally adds
 // ---------------------------------------------------------------------------
 
-// With one replica per XCD (`local`), every add to a replica comes from
-// the CUs behind one L2, so a workgroup-scope atomic -- performed in that
-// L2 -- is exact; device-scope atomics would be forwarded past the
-// XCD-private L2 to memory.  Kernel completion writes the L2s back before
-// the reduction kernel reads the replicas.
-template <bool FIXED>
-__device__ __forceinline__ void tally_add(void *base, int64_t idx, double v, double scale, bool local) {
-  if constexpr (FIXED) {
-    unsigned long long *a = reinterpret_cast<unsigned long long *>(base) + idx;
-    const unsigned long long q = (unsigned long long)__double2ll_rn(v * scale);
-    if (local) __hip_atomic_fetch_add(a, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else __hip_atomic_fetch_add(a, q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    double *a = reinterpret_cast<double *>(base) + idx;
-    if (local) __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Per-workgroup LDS cache of sector tallies.  A slot is claimed by the
+// first sector that hashes to it and keeps that sector for the rest of the
+// launch; flushes of a claimed sector stay in LDS and reach the global
+// replica once, when the workgroup finishes.  Every photon starts at a light
+// source, so the slots fill with the sectors next to the sources, which
+// take the most flushes.
+template <bool FIXED, int WS> struct SectorCache {
+  int *tag;                  // [S] sector id or -1
+  unsigned long long *val;   // [S][WS] FP64 bits or fixed point
+  int nslots;                // 0: cache off
+  __device__ __forceinline__ int slot_of(int sect) const {
+    return (int)(((uint64_t)((uint32_t)sect * 0x9E3779B1u) * (uint32_t)nslots) >> 32);
   }
-}
-
-// Per-workgroup LDS tally cache.  A slot is claimed by the first element
-// that hashes to it and keeps that element for the rest of the launch; adds
-// to a claimed element stay in LDS and are flushed to the global replica
-// once, when the workgroup finishes.  Every photon starts at a light source,
-// so the slots fill with the elements next to the sources, which are the
-// ones that take the most adds and would otherwise contend on one address.
-template <bool FIXED, int W> struct LdsTally {
-  int *tag;      // [S] element id or -1
-  void *val;     // [S * W] doubles or fixed-point int64
-  int mask;      // S - 1, or -1 when the cache is off
-  __device__ __forceinline__ int slot_of(int el) const {
-    return (int)(((uint32_t)el * 0x9E3779B1u) >> 8) & mask;
-  }
-  // true when the add went into LDS
-  __device__ __forceinline__ bool add(int el, double v0, double v1, double scale) {
-    if (mask < 0) return false;
-    const int sl = slot_of(el);
+  // moves the lane buffer words buf[j * stride] of `sect` into the sector's
+  // slot (clearing them); false, leaving them, when the slot belongs to
+  // another sector or the cache is off
+  template <class R>
+  __device__ __forceinline__ bool add_from(int sect, R *buf, int stride, double scale) {
+    if (nslots == 0) return false;
+    const int sl = slot_of(sect);
     int t = __hip_atomic_load(&tag[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (t == -1) {
-      const int old = atomicCAS(&tag[sl], -1, el);
-      t = old == -1 ? el : old;
+      const int old = atomicCAS(&tag[sl], -1, sect);
+      t = old == -1 ? sect : old;
     }
-    if (t != el) return false;
-    if constexpr (FIXED) {
-      unsigned long long *a = reinterpret_cast<unsigned long long *>(val) + (size_t)sl * W;
-      atomicAdd(a, (unsigned long long)__double2ll_rn(v0 * scale));
-      if (W == 2) atomicAdd(a + 1, (unsigned long long)__double2ll_rn(v1 * scale));
-    } else {
-      double *a = reinterpret_cast<double *>(val) + (size_t)sl * W;
-      atomicAdd(a, v0);
-      if (W == 2) atomicAdd(a + 1, v1);
+    if (t != sect) return false;
+#pragma unroll 4
+    for (int j = 0; j < WS; ++j) {
+      const R v = buf[j * stride];
+      buf[j * stride] = R(0);
+      if (v == R(0)) continue;
+      if constexpr (FIXED) atomicAdd(val + (size_t)sl * WS + j, (unsigned long long)__double2ll_rn((double)v * scale));
+      else atomicAdd(reinterpret_cast<double *>(val) + (size_t)sl * WS + j, (double)v);
     }
     return true;
   }
@@ -213,12 +202,15 @@ template <class R> struct Photon {
 template <int NF, class R>
 __device__ __forceinline__ void load_elem(const Elem<NF, R> *__restrict__ E, int el,
                                           typename Vec4T<R>::type (&pl)[NF],
-                                          typename Vec4T<R>::type &opt, i4 &nb) {
+                                          typename Vec4T<R>::type &opt, i4 &nb,
+                                          typename Vec2T<R>::type (&nbo)[NF]) {
   const Elem<NF, R> *e = E + el;
 #pragma unroll
   for (int f = 0; f < NF; ++f) pl[f] = e->pl[f];
   opt = e->opt;
   nb = e->nb;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) nbo[f] = e->nbo[f];
 }
 
 // Fresnel transmission/reflection at a face with outward unit normal n
@@ -447,20 +439,44 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, SeqDraws &rs, const 
 // up to 1024 lanes (one large LDS tally cache per CU); FP64 kernels need more
 // registers per lane and stay at 256.
 template <class R> struct MaxBlock { static constexpr int value = std::is_same<R, float>::value ? 1024 : 256; };
+// waves per SIMD the register allocation must leave room for: 4 (128 VGPRs)
+// for FP32, 2 (256) for FP64
+template <class R> struct MinWaves { static constexpr int value = std::is_same<R, float>::value ? 4 : 2; };
 
+// Boundary tally words written when a photon leaves through face ib, as
+// offsets into its [EBR, EBI, DEBR, DEBI] slots.
+template <int D, bool FD> struct ExitWords;
+template <> struct ExitWords<3, false> { static constexpr int n = 1; static constexpr int off[1] = {0}; };
+template <> struct ExitWords<3, true> { static constexpr int n = 2; static constexpr int off[2] = {0, 1}; };
+template <> struct ExitWords<2, false> { static constexpr int n = 2; static constexpr int off[2] = {0, 2}; };
+template <> struct ExitWords<2, true> { static constexpr int n = 4; static constexpr int off[4] = {0, 1, 2, 3}; };
+
+// Tally path.  Elements are numbered along a space-filling curve, so a
+// photon usually makes several visits in a row inside one sector of eight
+// consecutive elements.  Each lane sums its visits into a private LDS
+// buffer for its current sector; when the photon's visits move to another
+// sector, the buffer is flushed as one run of 8W words: into the
+// workgroup's sector cache if the sector holds a slot there, else into the
+// wave's run queue.  A full queue is drained with eight lanes per sector, so
+// each run costs one coalesced atomic request on one 64-byte line.  Device
+// atomics are served per line request (about 24 G requests/s on MI355X
+// whatever the word type), which makes requests, not words, the budget.
 template <int D, class R, bool FD, bool FIXED>
-__global__ void __launch_bounds__(MaxBlock<R>::value) walk_kernel(const WalkParams P) {
+__global__ void __launch_bounds__(MaxBlock<R>::value)
+    __attribute__((amdgpu_waves_per_eu(MinWaves<R>::value))) walk_kernel(const WalkParams P) {
   constexpr int NF = D + 1;
+  constexpr int W = FD ? 2 : 1;
+  typedef ExitWords<D, FD> XW;
   typedef typename Vec4T<R>::type V4;
   const Elem<NF, R> *__restrict__ E = reinterpret_cast<const Elem<NF, R> *>(P.elems);
   const bool local = P.xcc_replicas != 0;
   const int rep = local ? (xcc_id() % P.nrep) : (int)(blockIdx.x % (unsigned)P.nrep);
-  void *const etally = reinterpret_cast<char *>(P.etally) + (size_t)rep * P.ne * (FD ? 2 : 1) * 8;
+  void *const etally = reinterpret_cast<char *>(P.etally) + (size_t)rep * P.ne * W * 8;
   void *const btally = reinterpret_cast<char *>(P.btally) + (size_t)rep * P.nb * 4 * 8;
   const int lane = (int)__lane_id();
 
   Photon<R> p;
-  uint32_t plo = 0, phi = 0;   // photon id: the key of its random stream
+  WalkRng rs{0u, 0u, 0u, 0u};   // the walk's random stream of the lane's photon
   bool alive = false;
   // absorbed weight accumulates per lane; the rare end-of-photon sums go to
   // per-wave LDS slots [exited, killed, gained, lost, loss count]
@@ -471,21 +487,109 @@ __global__ void __launch_bounds__(MaxBlock<R>::value) walk_kernel(const WalkPara
 #pragma unroll
     for (int q = 0; q < 5; ++q) s_led[wid][q] = 0.0;
   }
-  constexpr int W = FD ? 2 : 1;
+  constexpr int WS = kSectorElems * W;   // tally words per sector
   extern __shared__ __align__(16) unsigned char s_dyn[];
-  LdsTally<FIXED, W> cache{reinterpret_cast<int *>(s_dyn),
-                           s_dyn + (size_t)(P.lds_slots > 0 ? P.lds_slots : 0) * 4,
-                           P.lds_slots > 0 ? P.lds_slots - 1 : -1};
-  for (int i = threadIdx.x; i < P.lds_slots; i += blockDim.x) {
-    cache.tag[i] = -1;
-    if constexpr (FIXED) {
-      for (int w = 0; w < W; ++w) reinterpret_cast<unsigned long long *>(cache.val)[(size_t)i * W + w] = 0ull;
-    } else {
-      for (int w = 0; w < W; ++w) reinterpret_cast<double *>(cache.val)[(size_t)i * W + w] = 0.0;
-    }
-  }
+  const int S = P.lds_slots > 0 ? P.lds_slots : 0;
+  const int Q = P.queue_len;
+  const WalkLds lay(W, (int)sizeof(R), S, Q, (int)blockDim.x);
+  SectorCache<FIXED, WS> cache{reinterpret_cast<int *>(s_dyn + lay.ctag),
+                               reinterpret_cast<unsigned long long *>(s_dyn + lay.cval), S};
+  R *const qv = reinterpret_cast<R *>(s_dyn + lay.qval) + (size_t)wid * Q * WS;
+  int *const qrun = reinterpret_cast<int *>(s_dyn + lay.qrun) + (size_t)wid * Q;
+  R *const sbuf = reinterpret_cast<R *>(s_dyn + lay.sbuf) + threadIdx.x;   // word j at [j * blockDim]
+  const int bstride = (int)blockDim.x;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) cache.tag[i] = -1;
+  for (int i = threadIdx.x; i < S * WS; i += blockDim.x) cache.val[i] = 0ull;
+#pragma unroll
+  for (int j = 0; j < WS; ++j) sbuf[j * bstride] = R(0);
   __syncthreads();
   uint64_t steps = 0;   // wave-uniform
+  int qn = 0;           // wave-uniform: runs in the queue
+  int sect = -1;        // sector held in the lane's buffer
+
+  // Add to the replica.  Word index: element word el*W + w, or boundary
+  // word 0x80000000 | ib*4 + j.  With one replica per XCD (`local`), every
+  // add to a replica comes from the CUs behind one L2, so a workgroup-scope
+  // atomic is exact.  Kernel completion writes the L2s back before the
+  // reduction kernel reads the replicas.
+  auto global_add = [&](uint32_t idx, unsigned long long bits) {
+    void *base = (idx & 0x80000000u) ? btally : etally;
+    const int64_t i = (int64_t)(idx & 0x7fffffffu);
+    if constexpr (FIXED) {
+      unsigned long long *a = reinterpret_cast<unsigned long long *>(base) + i;
+      if (local) __hip_atomic_fetch_add(a, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else __hip_atomic_fetch_add(a, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      double *a = reinterpret_cast<double *>(base) + i;
+      const double v = __longlong_as_double((long long)bits);
+      if (local) __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  };
+  auto to_bits = [&](double v) -> unsigned long long {
+    if constexpr (FIXED) return (unsigned long long)__double2ll_rn(v * P.fix_scale);
+    else return (unsigned long long)__double_as_longlong(v);
+  };
+  // converged: every lane of the wave calls these together
+  auto drain = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    constexpr int per = 64 / WS;   // runs per wave instruction
+    const int jr = lane / WS, jw = lane % WS;
+    for (int r0 = 0; r0 < qn; r0 += per) {
+      const int r = r0 + jr;
+      if (r < qn) {
+        const R v = qv[r * WS + jw];
+        if (v != R(0)) global_add((uint32_t)(qrun[r] * WS + jw), to_bits((double)v));
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    qn = 0;
+  };
+  // hand the lane's buffer (sector `sect`) to the cache or the queue if
+  // `go`, and clear it; words move LDS to LDS one at a time
+  auto flush_sector = [&](bool go) {
+    const bool need = go && !cache.add_from(sect, sbuf, bstride, P.fix_scale);
+    uint64_t m = __ballot(need);
+    if (m == 0) return;
+    if (Q == 0) {
+      if (need) {
+#pragma unroll 4
+        for (int j = 0; j < WS; ++j) {
+          const R w = sbuf[j * bstride];
+          sbuf[j * bstride] = R(0);
+          if (w != R(0)) global_add((uint32_t)(sect * WS + j), to_bits((double)w));
+        }
+      }
+      return;
+    }
+    while (m) {
+      if (qn == Q) drain();
+      const int room = Q - qn;
+      const bool mine = (m >> lane) & 1ull;
+      const int rk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      const bool take = mine && rk < room;
+      const uint64_t t = __ballot(take);
+      if (take) {
+        const int r = qn + rk;
+        qrun[r] = sect;
+#pragma unroll 4
+        for (int j = 0; j < WS; ++j) {
+          qv[r * WS + j] = sbuf[j * bstride];
+          sbuf[j * bstride] = R(0);
+        }
+      }
+      qn += __popcll(t);
+      m &= ~t;
+    }
+  };
+
+  // parked tally work of the last step
+  int pe_el = -1;         // element whose visit ended, with its deposits
+  R pe0 = R(0), pe1 = R(0);
+  int pb_ib = -1;         // boundary face the photon left through, with its words
+  R pb[XW::n];
+#pragma unroll
+  for (int j = 0; j < XW::n; ++j) pb[j] = R(0);
 
   // wave-private photon id pool [pool_next, pool_end)
   uint64_t pool_next = 0, pool_end = 0;
@@ -519,11 +623,11 @@ __global__ void __launch_bounds__(MaxBlock<R>::value) walk_kernel(const WalkPara
         const bool invalid = !alive && pid >= P.n_end;
         if (__ballot(invalid)) exhausted = true;
         if (!alive && !invalid) {
-          plo = (uint32_t)pid;
-          phi = (uint32_t)(pid >> 32);
+          const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
           SeqDraws cs{plo, phi, 0u, 0u, 0u, 0u};
           create_photon<D, R>(p, cs, P);
           p.prop = -m_log(Unif<R>::open(cs.next(P))) / E[p.el].opt.y;
+          rs.seed(plo, phi, P);
           alive = true;
         }
       }
@@ -531,234 +635,246 @@ __global__ void __launch_bounds__(MaxBlock<R>::value) walk_kernel(const WalkPara
     const uint64_t live = __ballot(alive);
     if (live == 0) break;
     steps += __popcll(live);
-    StepDraws rs;
-    rs.begin(plo, phi, (uint32_t)p.nsteps, P);   // converged: all lanes at once
-    if (!alive) continue;
 
-    // ---- one element step ----
-    // the element record is fetched per step rather than carried across
-    // iterations: it keeps the loop state small (more waves per SIMD) and a
-    // repeated fetch of the same record hits in the vector L1
-    V4 pl[NF];
-    V4 opt;
-    i4 nb;
-    load_elem<NF, R>(E, p.el, pl, opt, nb);
-    ++p.nsteps;
-    // one element visit (entry to exit, scatterings inside included) adds
-    // its deposits once: to the workgroup's LDS cache, else to the replica
-    auto flush_visit = [&]() {
-      if constexpr (!FD) {
-        if (p.acc0 != R(0) && !cache.add(p.el, (double)p.acc0, 0.0, P.fix_scale))
-          tally_add<FIXED>(etally, p.el, (double)p.acc0, P.fix_scale, local);
-      } else {
-        if (!cache.add(p.el, (double)p.acc0, (double)p.acc1, P.fix_scale)) {
-          tally_add<FIXED>(etally, 2 * (int64_t)p.el, (double)p.acc0, P.fix_scale, local);
-          tally_add<FIXED>(etally, 2 * (int64_t)p.el + 1, (double)p.acc1, P.fix_scale, local);
+    // ---- one element step (returns early where the reference's loop does) ----
+    auto step = [&]() {
+      // the element record is fetched per step rather than carried across
+      // iterations: it keeps the loop state small (more waves per SIMD) and a
+      // repeated fetch of the same record hits in the caches
+      V4 pl[NF];
+      V4 opt;
+      i4 nb;
+      typename Vec2T<R>::type nbo[NF];
+      load_elem<NF, R>(E, p.el, pl, opt, nb, nbo);
+      ++p.nsteps;
+      // one element visit (entry to exit, scatterings inside included) adds
+      // its deposits once
+      auto end_visit = [&]() {
+        if (FD || p.acc0 != R(0)) {
+          pe_el = p.el;
+          pe0 = p.acc0;
+          pe1 = p.acc1;
         }
-      }
-      p.acc0 = R(0);
-      p.acc1 = R(0);
-    };
-    // exit face: smallest positive plane distance over faces moving outward,
-    // the current face excluded (MC3D.hpp:1020-1056)
-    R bn = R(1), bc = R(0);
-    int bf = -1, code = kNbNone;
-    R nrm[3] = {R(0), R(0), R(0)};
-    bool out_cur = false;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      R c = pl[f].x * p.d[0] + pl[f].y * p.d[1];
-      R q = pl[f].x * p.x[0] + pl[f].y * p.x[1];
-      if (D == 3) {
-        c += pl[f].z * p.d[2];
-        q += pl[f].z * p.x[2];
-      }
-      const R num = fmax(pl[f].w - q, R(0));
-      const bool cur = f == p.face;
-      out_cur = cur ? c > Tol<R>::out : out_cur;
-      if (!cur && c > R(0) && num * bc < bn * c) {
-        bn = num;
-        bc = c;
-        bf = f;
-        code = nb[f];
-        nrm[0] = pl[f].x;
-        nrm[1] = pl[f].y;
-        nrm[2] = D == 3 ? pl[f].z : R(0);
-      }
-    }
-    if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
-      // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
-      flush_visit();
-      atomicAdd(&s_led[wid][3], (double)p.w);
-      atomicAdd(&s_led[wid][4], 1.0);
-      alive = false;
-      continue;
-    }
-    const R dist = bn / bc;
-    const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
-#pragma unroll
-    for (int k = 0; k < D; ++k) p.x[k] += p.d[k] * ds;
-
-    const R mua = opt.x, mus = opt.y, nidx = opt.z;
-    // element tally and attenuation (MC3D.hpp:1479-1526)
-    constexpr bool kF64 = std::is_same<R, double>::value;
-    // FP64: the reference's att = exp(-mua ds) and (1 - att) w.  FP32: the
-    // deposit -w expm1(-mua ds) is subtracted from w exactly, so the weight
-    // ledger closes to rounding of its FP64 sums.
-    const R em = kF64 ? R(0) : m_expm1(-mua * ds);
-    const R att = kF64 ? m_exp(-mua * ds) : R(1) + em;
-    const R dabs = kF64 ? p.w - p.w * att : -p.w * em;
-    if constexpr (!FD) {
-      const R dep = mua > R(0) ? (kF64 ? (R(1) - att) * p.w : dabs) : p.w * ds;
-      p.acc0 += dep;
-    } else {
-      // w e^{-i phi} (1 - e^{-(mua + i k) ds}), MC3D.hpp:1519-1522, with the
-      // phase advance b = k ds taken from half-angle terms
-      const R b = R(P.kfac) * nidx * ds;
-      R sh, ch;
-      m_sincos(R(0.5) * b, &sh, &ch);
-      const R sb = R(2) * sh * ch, ver = R(2) * sh * sh, cb = R(1) - ver;
-      const R c1 = p.cph * cb - p.sph * sb, s1 = p.sph * cb + p.cph * sb;
-      R er, ei;
-      if constexpr (kF64) {
-        er = p.w * (p.cph - c1 * att);
-        ei = p.w * (-p.sph + s1 * att);
-      } else {
-        const R qq = -em + att * ver;   // 1 - att cos(b) without cancellation
-        er = p.w * (p.cph * qq + att * p.sph * sb);
-        ei = p.w * (-p.sph * qq + att * p.cph * sb);
-      }
-      p.acc0 += er;
-      p.acc1 += ei;
-      if constexpr (kF64) {
-        p.cph = c1;
-        p.sph = s1;
-      } else {
-        const R rn = R(1.5) - R(0.5) * (c1 * c1 + s1 * s1);   // keep |e^{i phi}| = 1
-        p.cph = c1 * rn;
-        p.sph = s1 * rn;
-      }
-    }
-    l_abs += (double)dabs;
-    p.w = kF64 ? p.w * att : p.w - dabs;
-    p.prop -= ds;
-
-    if (p.prop <= R(0)) {
-      // scattering event (MC3D.hpp:1634-1636), then a new free path
-      if (mus > R(0)) {
-        if constexpr (D == 3) scatter3<R>(p, opt.w, rs, P);
-        else scatter2<R>(p, opt.w, rs, P);
-      }
-      p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus;
-      continue;
-    }
-
-    if (code < 0) {
-      if (code == kNbNone) {
-        flush_visit();
+        p.acc0 = R(0);
+        p.acc1 = R(0);
+      };
+      auto lose = [&]() {
+        end_visit();
         atomicAdd(&s_led[wid][3], (double)p.w);
         atomicAdd(&s_led[wid][4], 1.0);
         alive = false;
-        continue;
-      }
-      // boundary (MC3D.hpp:1536-1567)
-      const int ib = -1 - code;
-      const BndRec b = P.bnd[ib];
-      if (b.kind == kBndMirror) {
-        R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
-        if (D == 3) cd += p.d[2] * nrm[2];
+      };
+      // exit face: smallest positive plane distance over faces moving outward,
+      // the current face excluded (MC3D.hpp:1020-1056)
+      R bn = R(1), bc = R(0);
+      int bf = -1, code = kNbNone;
+      R nrm[3] = {R(0), R(0), R(0)};
+      R mus_n = R(0), n_n = R(0);
+      bool out_cur = false;
 #pragma unroll
-        for (int k = 0; k < D; ++k) p.d[k] -= R(2) * cd * nrm[k];
-        p.face = bf;
-        continue;
-      }
-      if (b.kind == kBndFresnel && fresnel<D, R>(p, nrm, nidx, R(b.bcn), false, rs, P)) {
-        p.face = bf;
-        continue;
-      }
-      const size_t bo = (size_t)ib * 4;
-      if (!FD) {
-        tally_add<FIXED>(btally, bo, (double)p.w, P.fix_scale, local);
-      } else {
-        tally_add<FIXED>(btally, bo, (double)(p.w * p.cph), P.fix_scale, local);
-        tally_add<FIXED>(btally, bo + 1, (double)(-p.w * p.sph), P.fix_scale, local);
-      }
-      if constexpr (D == 2) {
-        // exitance weighted by 1/(dir . outward normal), MC2D.hpp:1410-1437
-        const R dn = p.d[0] * nrm[0] + p.d[1] * nrm[1];
-        if (!FD) {
-          tally_add<FIXED>(btally, bo + 2, (double)(p.w / dn), P.fix_scale, local);
-        } else {
-          tally_add<FIXED>(btally, bo + 2, (double)(p.w * p.cph / dn), P.fix_scale, local);
-          tally_add<FIXED>(btally, bo + 3, (double)(-p.w * p.sph / dn), P.fix_scale, local);
+      for (int f = 0; f < NF; ++f) {
+        R c = pl[f].x * p.d[0] + pl[f].y * p.d[1];
+        R q = pl[f].x * p.x[0] + pl[f].y * p.x[1];
+        if (D == 3) {
+          c += pl[f].z * p.d[2];
+          q += pl[f].z * p.x[2];
+        }
+        const R num = fmax(pl[f].w - q, R(0));
+        const bool cur = f == p.face;
+        out_cur = cur ? c > Tol<R>::out : out_cur;
+        if (!cur && c > R(0) && num * bc < bn * c) {
+          bn = num;
+          bc = c;
+          bf = f;
+          code = nb[f];
+          nrm[0] = pl[f].x;
+          nrm[1] = pl[f].y;
+          nrm[2] = D == 3 ? pl[f].z : R(0);
+          mus_n = nbo[f].x;
+          n_n = nbo[f].y;
         }
       }
-      flush_visit();
-      atomicAdd(&s_led[wid][0], (double)p.w);
-      alive = false;
-      continue;
-    }
-
-    // interior face: enter the neighbour (MC3D.hpp:1570-1631)
-    const int next = code >> 3;
-    const bool back = (code >> 2) & 1;
-    const int entry = code & 3;
-    const R mus_c = mus, n_c = nidx;
-    const V4 opt_n = E[next].opt;
-    const R mus_n = opt_n.y, n_n = opt_n.z;
-    bool reflected = false;
-    if (mus_c <= R(0) && mus_n > R(0)) {
-      // vacuum -> scattering medium: fresh free path
-      p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus_n;
-    }
-    if (p.w < R(P.weight0)) {
-      // Russian roulette at the crossing (MC3D.hpp:1597-1604)
-      if (Unif<R>::closed(rs.next(P)) > R(P.chance)) {
-        flush_visit();
-        atomicAdd(&s_led[wid][1], (double)p.w);
-        alive = false;
-        continue;
+      if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
+        // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
+        lose();
+        return;
       }
-      const R wn = p.w / R(P.chance);
-      atomicAdd(&s_led[wid][2], (double)wn - (double)p.w);
-      p.w = wn;
+      const R dist = bn / bc;
+      const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
+#pragma unroll
+      for (int k = 0; k < D; ++k) p.x[k] += p.d[k] * ds;
+
+      const R mua = opt.x, mus = opt.y, nidx = opt.z;
+      // element tally and attenuation (MC3D.hpp:1479-1526)
+      constexpr bool kF64 = std::is_same<R, double>::value;
+      // FP64: the reference's att = exp(-mua ds) and (1 - att) w.  FP32: the
+      // deposit -w expm1(-mua ds) is subtracted from w exactly, so the weight
+      // ledger closes to rounding of its FP64 sums.
+      const R em = kF64 ? R(0) : m_expm1(-mua * ds);
+      const R att = kF64 ? m_exp(-mua * ds) : R(1) + em;
+      const R dabs = kF64 ? p.w - p.w * att : -p.w * em;
+      if constexpr (!FD) {
+        const R dep = mua > R(0) ? (kF64 ? (R(1) - att) * p.w : dabs) : p.w * ds;
+        p.acc0 += dep;
+      } else {
+        // w e^{-i phi} (1 - e^{-(mua + i k) ds}), MC3D.hpp:1519-1522, with the
+        // phase advance b = k ds taken from half-angle terms
+        const R b = R(P.kfac) * nidx * ds;
+        R sh, ch;
+        m_sincos(R(0.5) * b, &sh, &ch);
+        const R sb = R(2) * sh * ch, ver = R(2) * sh * sh, cb = R(1) - ver;
+        const R c1 = p.cph * cb - p.sph * sb, s1 = p.sph * cb + p.cph * sb;
+        R er, ei;
+        if constexpr (kF64) {
+          er = p.w * (p.cph - c1 * att);
+          ei = p.w * (-p.sph + s1 * att);
+        } else {
+          const R qq = -em + att * ver;   // 1 - att cos(b) without cancellation
+          er = p.w * (p.cph * qq + att * p.sph * sb);
+          ei = p.w * (-p.sph * qq + att * p.cph * sb);
+        }
+        p.acc0 += er;
+        p.acc1 += ei;
+        if constexpr (kF64) {
+          p.cph = c1;
+          p.sph = s1;
+        } else {
+          const R rn = R(1.5) - R(0.5) * (c1 * c1 + s1 * s1);   // keep |e^{i phi}| = 1
+          p.cph = c1 * rn;
+          p.sph = s1 * rn;
+        }
+      }
+      l_abs += (double)dabs;
+      p.w = kF64 ? p.w * att : p.w - dabs;
+      p.prop -= ds;
+
+      if (p.prop <= R(0)) {
+        // scattering event (MC3D.hpp:1634-1636), then a new free path
+        if (mus > R(0)) {
+          if constexpr (D == 3) scatter3<R>(p, opt.w, rs, P);
+          else scatter2<R>(p, opt.w, rs, P);
+        }
+        p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus;
+        return;
+      }
+
+      if (code < 0) {
+        if (code == kNbNone) {
+          lose();
+          return;
+        }
+        // boundary (MC3D.hpp:1536-1567)
+        const int ib = -1 - code;
+        const BndRec b = P.bnd[ib];
+        if (b.kind == kBndMirror) {
+          R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
+          if (D == 3) cd += p.d[2] * nrm[2];
+#pragma unroll
+          for (int k = 0; k < D; ++k) p.d[k] -= R(2) * cd * nrm[k];
+          p.face = bf;
+          return;
+        }
+        if (b.kind == kBndFresnel && fresnel<D, R>(p, nrm, nidx, R(b.bcn), false, rs, P)) {
+          p.face = bf;
+          return;
+        }
+        // exitance (MC3D.hpp:1556-1564); 2D adds it weighted by
+        // 1/(dir . outward normal) (MC2D.hpp:1410-1437)
+        pb_ib = ib;
+        pb[0] = FD ? p.w * p.cph : p.w;
+        if constexpr (FD) pb[1] = -p.w * p.sph;
+        if constexpr (D == 2) {
+          const R dn = p.d[0] * nrm[0] + p.d[1] * nrm[1];
+          if constexpr (!FD) {
+            pb[1] = p.w / dn;
+          } else {
+            pb[2] = p.w * p.cph / dn;
+            pb[3] = -p.w * p.sph / dn;
+          }
+        }
+        end_visit();
+        atomicAdd(&s_led[wid][0], (double)p.w);
+        alive = false;
+        return;
+      }
+
+      // interior face: enter the neighbour (MC3D.hpp:1570-1631)
+      const int next = code >> 3;
+      const bool back = (code >> 2) & 1;
+      const int entry = code & 3;
+      const R mus_c = mus, n_c = nidx;
+      if (mus_c <= R(0) && mus_n > R(0)) {
+        // vacuum -> scattering medium: fresh free path
+        p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus_n;
+      }
+      if (p.w < R(P.weight0)) {
+        // Russian roulette at the crossing (MC3D.hpp:1597-1604)
+        if (Unif<R>::closed(rs.next(P)) > R(P.chance)) {
+          end_visit();
+          atomicAdd(&s_led[wid][1], (double)p.w);
+          alive = false;
+          return;
+        }
+        const R wn = p.w / R(P.chance);
+        atomicAdd(&s_led[wid][2], (double)wn - (double)p.w);
+        p.w = wn;
+      }
+      if (n_c != n_n && fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P)) {
+        p.face = bf;
+        return;
+      }
+      if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= mus_c / mus_n;
+      if (!back) {
+        // the neighbour does not list this element: back-lookup failure
+        lose();
+        return;
+      }
+      end_visit();
+      p.el = next;
+      p.face = entry;
+    };
+    const bool stepped = alive;
+    if (alive) step();
+
+    // ---- commit the parked tally work (converged) ----
+    {
+      const int es = pe_el >= 0 ? pe_el / kSectorElems : -1;
+      flush_sector(es >= 0 && sect >= 0 && es != sect);
+      if (es >= 0) {
+        const int k = pe_el - es * kSectorElems;
+        sbuf[(k * W) * bstride] += pe0;
+        if constexpr (FD) sbuf[(k * W + 1) * bstride] += pe1;
+        sect = es;
+      }
+      if constexpr (FIXED) {
+        // a buffer sum must hold one photon's deposits only, so fixed-point
+        // results do not depend on which photons shared a lane
+        const bool ended = stepped && !alive && sect >= 0;
+        flush_sector(ended);
+        if (ended) sect = -1;
+      }
+      if (pb_ib >= 0) {
+#pragma unroll
+        for (int j = 0; j < XW::n; ++j)
+          global_add(0x80000000u | (uint32_t)(pb_ib * 4 + XW::off[j]), to_bits((double)pb[j]));
+      }
+      pe_el = -1;
+      pb_ib = -1;
     }
-    if (n_c != n_n) reflected = fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P);
-    if (reflected) {
-      p.face = bf;
-      continue;
-    }
-    if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= mus_c / mus_n;
-    if (!back) {
-      // the neighbour does not list this element: back-lookup failure
-      flush_visit();
-      atomicAdd(&s_led[wid][3], (double)p.w);
-      atomicAdd(&s_led[wid][4], 1.0);
-      alive = false;
-      continue;
-    }
-    flush_visit();
-    p.el = next;
-    p.face = entry;
   }
+  flush_sector(sect >= 0);
+  if (Q > 0) drain();
 
   // ---- flush the counters: one set of global atomics per wave ----
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) l_abs += __shfl_xor(l_abs, off);
   __syncthreads();
-  // ---- flush the LDS tally cache into this workgroup's replica ----
-  for (int i = threadIdx.x; i < P.lds_slots; i += blockDim.x) {
-    const int el = cache.tag[i];
-    if (el < 0) continue;
-    for (int w = 0; w < W; ++w) {
-      if constexpr (FIXED) {
-        const unsigned long long q = reinterpret_cast<unsigned long long *>(cache.val)[(size_t)i * W + w];
-        if (q) atomicAdd(reinterpret_cast<unsigned long long *>(etally) + (size_t)el * W + w, q);
-      } else {
-        const double v = reinterpret_cast<double *>(cache.val)[(size_t)i * W + w];
-        if (v != 0.0) unsafeAtomicAdd(reinterpret_cast<double *>(etally) + (size_t)el * W + w, v);
-      }
-    }
+  // ---- flush the sector cache into this workgroup's replica: one sector
+  // per WS consecutive lanes ----
+  for (int i = threadIdx.x; i < S * WS; i += blockDim.x) {
+    const int sc = cache.tag[i / WS];
+    const unsigned long long bits = cache.val[i];
+    if (sc >= 0 && bits) global_add((uint32_t)(sc * WS + i % WS), bits);
   }
   if (lane == 0) {
     unsafeAtomicAdd(&P.ledger[0], l_abs);

@@ -16,9 +16,15 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef int i4 __attribute__((ext_vector_type(4)));
 
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
 template <class R> struct Vec4T;
 template <> struct Vec4T<float> { typedef f4 type; };
 template <> struct Vec4T<double> { typedef d4 type; };
+template <class R> struct Vec2T;
+template <> struct Vec2T<float> { typedef f2 type; };
+template <> struct Vec2T<double> { typedef d2 type; };
 
 // Neighbour code of one element face (reference HN, cpp/3d/MC3D.hpp:1025,
 // 1616-1628, folded with the entry-face back-lookup):
@@ -30,12 +36,19 @@ constexpr int32_t kNbNone = INT32_MIN;
 constexpr int64_t kMaxElements = (int64_t)1 << 28;
 
 // Face f: outward unit normal n and offset d with n.x <= d inside.
-// opt = (mua, mus, n, g).  Padding keeps records on 64-byte boundaries.
+// opt = (mua, mus, n, g).  nbo[f] = (mus, n) of the element across face f
+// (zero on the boundary): a crossing needs them (free-path rescale and the
+// Fresnel test, MC3D.hpp:1570-1613) and would otherwise gather them from the
+// neighbour's record.  FP32 tetrahedra fill exactly one 128-byte line; the
+// FP64 record is two lines.
 template <int NF, class R> struct alignas(64) Elem {
   typename Vec4T<R>::type pl[NF];
   typename Vec4T<R>::type opt;
   i4 nb;
+  typename Vec2T<R>::type nbo[NF];
 };
+static_assert(sizeof(Elem<4, float>) == 128, "FP32 tetrahedron record is one cache line");
+static_assert(sizeof(Elem<4, double>) == 256, "FP64 tetrahedron record is two cache lines");
 
 // Boundary element kinds for the walk (BCType, MC3D.hpp:1536-1567)
 enum BndKind : int32_t {
@@ -72,8 +85,33 @@ struct alignas(16) SrcRec {
   int32_t mother, face, kind, ib;
 };
 
-// Tally layout: nrep replicas of [Ne][cw ? 1 : 2] element and [Nb][4]
-// boundary slots (EBR, EBI, DEBR, DEBI).
+// Tally sectors: eight consecutive elements (device order), i.e. 8*W
+// tally words, one 64-byte line of the replica in continuous-wave mode.
+constexpr int kSectorElems = 8;
+
+// Dynamic LDS of one walk workgroup, in bytes, in this order (each part
+// 16-byte aligned):
+//   sector cache values  [S][8W]  u64 (FP64 sums or fixed point)
+//   run queue values     [waves][Q][8W]  R
+//   lane sector buffers  [8W][block]     R
+//   sector cache tags    [S]  int32
+//   run queue sectors    [waves][Q]  int32
+struct WalkLds {
+  size_t cval, qval, sbuf, ctag, qrun, total;
+  __host__ __device__ WalkLds(int W, int rbytes, int S, int Q, int block) {
+    const size_t ws = (size_t)kSectorElems * W, waves = (size_t)block / 64;
+    auto up = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    cval = 0;
+    qval = up(cval + (size_t)S * ws * 8);
+    sbuf = up(qval + waves * (size_t)Q * ws * rbytes);
+    ctag = up(sbuf + ws * (size_t)block * rbytes);
+    qrun = up(ctag + (size_t)S * 4);
+    total = up(qrun + waves * (size_t)Q * 4);
+  }
+};
+
+// Tally layout: nrep replicas of [Ne rounded up to a sector][cw ? 1 : 2]
+// element and [Nb][4] boundary slots (EBR, EBI, DEBR, DEBI).
 struct WalkParams {
   const void *elems;
   const double *cdf;
@@ -83,7 +121,7 @@ struct WalkParams {
   void *btally;
   double *ledger;               // [5] absorbed, exited, killed, gained, lost
   unsigned long long *counters; // [4] photon dispenser, steps, loss, rechecks
-  int64_t ne, nb;
+  int64_t ne, nb;               // ne: element count rounded up to a whole sector
   uint64_t n_end;               // photons [*, n_end) of this run
   int32_t nsrc, nrep;
   uint32_t key0, key1, tag;
@@ -91,7 +129,8 @@ struct WalkParams {
   int32_t refill_min;
   int32_t chunk;
   int32_t xcc_replicas;         // 1: replica = XCD id, workgroup-scope (L2) atomics
-  int32_t lds_slots;            // LDS tally cache entries per workgroup (0 = none)
+  int32_t lds_slots;            // LDS sector-cache slots per workgroup (0 = none)
+  int32_t queue_len;            // LDS run-queue sectors per wave (0 = direct atomics)
   double phase0, cph0, sph0;    // initial phase and its cos/sin
   double kfac;                  // omega / c0: k = kfac * n
   double weight0, chance;
