@@ -76,7 +76,7 @@ typedef struct vmc_run_desc {
   int32_t block_size;    /* 0 = default (FP32 1024, FP64 256); multiple of 64   */
   int32_t blocks_per_cu; /* 0 = as many as fit                                 */
   int32_t replicas;      /* tally copies, 0 = one per XCD (8)                  */
-  int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 1 */
+  int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 6 */
   uint32_t stream_tag;   /* extra stream word: runs with different tags are
                             independent (batch instances use their index)      */
   int32_t lds_slots;     /* per-workgroup LDS tally cache entries (power of 2);

@@ -11,9 +11,8 @@
  *     with seed+t for OpenMP thread t, N/T photons per thread with the
  *     remainder on the last thread, tallies reduced in thread order
  *     (MC3D.hpp:1648-1748).  Pinned bit-for-bit against the reference engine.
- *   ORACLE_RNG_PHILOX -- photon i draws from its own Philox4x32-10 stream
- *     keyed by (seed, i): sequential words during creation, then a
- *     xoshiro128** generator started from one more Philox block
+ *   ORACLE_RNG_PHILOX -- photon i draws from its own xoshiro128** stream
+ *     started from a Philox4x32-10 block keyed by (seed, i)
  *     (PhiloxStream below).  This is the stream definition
  *     the HIP kernels use, so their FP64 path can be compared with this
  *     file photon-for-photon.
@@ -84,7 +83,6 @@ struct MtStream {
   std::mt19937 g;
   explicit MtStream(uint64_t s) : g((uint32_t)(s & 0xffffffffu)) {}
   void begin_photon(uint64_t) {}
-  void begin_walk() {}
   uint32_t next() { return (uint32_t)g(); }
 };
 
@@ -106,50 +104,36 @@ void philox_block(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t o
   out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* One stream per photon: Philox key = (seed lo, seed hi), counter =
-   (block, tag, photon lo, photon hi).  Photon creation and the first free
-   path read words 0, 1, 2, ... of blocks 0, 1, ... in order; the walk then
-   draws from xoshiro128** (Blackman and Vigna, "Scrambled linear
-   pseudorandom number generators", 2018: output rotl(s1 * 5, 7) * 9, state
-   update by xor/shift/rotate) started from the four words of Philox block
-   0x80000000 (an all-zero start is replaced by s0 = 1).  This is the HIP
-   kernels' definition. */
+/* One stream per photon: xoshiro128** (Blackman and Vigna, "Scrambled
+   linear pseudorandom number generators", 2018: output rotl(s1 * 5, 7) * 9,
+   state update by xor/shift/rotate) started from the four words of the
+   Philox4x32-10 block with counter (0x80000000, tag, photon lo, photon hi)
+   and key (seed lo, seed hi); an all-zero start is replaced by s0 = 1.
+   Creation and walk draw from it in the reference's order.  This is the
+   HIP kernels' definition. */
 struct PhiloxStream {
   uint32_t key[2];
   uint32_t tag;
-  uint64_t pid = 0;
-  uint32_t k = 0;
-  bool walking = false;
-  uint32_t buf[4];
   uint32_t xs[4];
   explicit PhiloxStream(uint64_t seed, uint32_t tag_ = 0) : tag(tag_) {
     key[0] = (uint32_t)seed; key[1] = (uint32_t)(seed >> 32);
   }
-  void begin_photon(uint64_t id) { pid = id; k = 0; walking = false; }
-  void begin_walk() {
-    uint32_t ctr[4] = {0x80000000u, tag, (uint32_t)pid, (uint32_t)(pid >> 32)};
+  void begin_photon(uint64_t id) {
+    uint32_t ctr[4] = {0x80000000u, tag, (uint32_t)id, (uint32_t)(id >> 32)};
     philox_block(ctr, key, xs);
     if ((xs[0] | xs[1] | xs[2] | xs[3]) == 0u) xs[0] = 1u;
-    walking = true;
   }
   static uint32_t rotl(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
   uint32_t next() {
-    if (walking) {
-      const uint32_t r = rotl(xs[1] * 5u, 7) * 9u;
-      const uint32_t t = xs[1] << 9;
-      xs[2] ^= xs[0];
-      xs[3] ^= xs[1];
-      xs[1] ^= xs[2];
-      xs[0] ^= xs[3];
-      xs[2] ^= t;
-      xs[3] = rotl(xs[3], 11);
-      return r;
-    }
-    if ((k & 3u) == 0) {
-      uint32_t ctr[4] = {k >> 2, tag, (uint32_t)pid, (uint32_t)(pid >> 32)};
-      philox_block(ctr, key, buf);
-    }
-    return buf[(k++) & 3u];
+    const uint32_t r = rotl(xs[1] * 5u, 7) * 9u;
+    const uint32_t t = xs[1] << 9;
+    xs[2] ^= xs[0];
+    xs[3] ^= xs[1];
+    xs[1] ^= xs[2];
+    xs[0] ^= xs[3];
+    xs[2] ^= t;
+    xs[3] = rotl(xs[3], 11);
+    return r;
   }
 };
 
@@ -813,10 +797,8 @@ struct Walker {
   void propagate(Packet &ph) {
     const bool d3 = m.dim == 3;
     double prop, dist = 0, ds;
-    bool first = true;
     for (;;) {
       prop = -log(open()) / m.mus[ph.curel];
-      if (first) { rng.begin_walk(); first = false; }
       for (;;) {
         int wf = d3 ? which_face3(ph, &dist) : which_face2(ph, &dist);
         if (wf == -1) { ++loss; led[4] += ph.weight; return; }

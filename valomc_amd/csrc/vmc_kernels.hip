@@ -53,32 +53,15 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
   o3 = c3;
 }
 
-// Stream of photon p (64-bit id) under key (seed) and tag:
-//  * photon creation and its first free path read words 0, 1, 2, ... of
-//    Philox blocks (0, tag, p), (1, tag, p), ... in order (SeqDraws);
-//  * the walk then draws from a xoshiro128** generator (Blackman & Vigna,
-//    "Scrambled linear pseudorandom number generators", 2018) whose 128-bit
-//    state is Philox block (0x80000000, tag, p).  A draw costs a few
-//    full-rate integer instructions and is made only in the branch that
-//    needs it, so lanes that cross a face without scattering pay nothing.
-//    Random 128-bit starting points in a 2^128 - 1 period make overlap of
-//    two photons' sequences negligible.
-struct SeqDraws {
-  uint32_t plo, phi, k, b1, b2, b3;
-  __device__ __forceinline__ uint32_t next(const WalkParams &P) {
-    uint32_t r;
-    if ((k & 3u) == 0) {
-      philox4x32_10(k >> 2, P.tag, plo, phi, P.key0, P.key1, r, b1, b2, b3);
-    } else {
-      r = b1;
-      b1 = b2;
-      b2 = b3;
-    }
-    ++k;
-    return r;
-  }
-};
-
+// Stream of photon p (64-bit id) under key (seed) and tag: a xoshiro128**
+// generator (Blackman & Vigna, "Scrambled linear pseudorandom number
+// generators", 2018) whose 128-bit state is the Philox4x32-10 block
+// (0x80000000, tag, p lo, p hi) under key (seed lo, seed hi).  Creation and
+// walk draw from it in the reference's order.  A draw costs a few full-rate
+// integer instructions and is made only in the branch that needs it, so
+// lanes that cross a face without scattering pay nothing; random 128-bit
+// starting points in a 2^128 - 1 period make overlap of two photons'
+// sequences negligible.
 struct WalkRng {
   uint32_t s0, s1, s2, s3;
   __device__ __forceinline__ void seed(uint32_t plo, uint32_t phi, const WalkParams &P) {
@@ -135,6 +118,39 @@ __device__ __forceinline__ void m_sincos(float x, float *s, float *c) { sincosf(
 __device__ __forceinline__ void m_sincos(double x, double *s, double *c) { sincos(x, s, c); }
 __device__ __forceinline__ void m_sincospi(float x, float *s, float *c) { sincospif(x, s, c); }
 __device__ __forceinline__ void m_sincospi(double x, double *s, double *c) { sincospi(x, s, c); }
+
+// Walk-loop math.  FP64 keeps the libm functions of the reference's
+// expressions.  FP32 uses the hardware transcendental instructions, whose
+// errors (about an ulp for log2/exp2, a few 1e-7 absolute for sin/cos of an
+// argument in revolutions) sit far below the FP32 transport's 1% budget:
+//   -log(u) = -log2(u) ln 2            (u in (0, 1])
+//   sin, cos of 2 pi u                 (u in revolutions, |u| < 256)
+//   expm1(x), x <= 0                   Taylor series to x^5 on (-1/16, 0]
+//                                      (relative error < 2e-9), else exp2 - 1
+__device__ __forceinline__ float w_neglog(float u) { return -__builtin_amdgcn_logf(u) * 0.693147180559945309f; }
+__device__ __forceinline__ double w_neglog(double u) { return -log(u); }
+__device__ __forceinline__ void w_sincos_rev(float u, float *s, float *c) {
+  *s = __builtin_amdgcn_sinf(u);
+  *c = __builtin_amdgcn_cosf(u);
+}
+__device__ __forceinline__ void w_sincos_rev(double u, double *s, double *c) { sincospi(2.0 * u, s, c); }
+__device__ __forceinline__ float w_expm1_neg(float x) {
+  const float p = x * (1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f)))));
+  const float e = __builtin_amdgcn_exp2f(x * 1.44269504088896341f) - 1.0f;
+  return x > -0.0625f ? p : e;
+}
+__device__ __forceinline__ double w_expm1_neg(double x) { return expm1(x); }
+// sin and cos of a small angle x in radians
+__device__ __forceinline__ void w_sincos_small(float x, float *s, float *c) {
+  w_sincos_rev(x * 0.159154943091895336f, s, c);
+}
+__device__ __forceinline__ void w_sincos_small(double x, double *s, double *c) { sincos(x, s, c); }
+// a / b and 1 / sqrt(x): hardware reciprocal (about an ulp) in FP32,
+// the reference's correctly rounded expressions in FP64
+__device__ __forceinline__ float w_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double w_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float w_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double w_rsqrt(double x) { return 1.0 / sqrt(x); }
 
 template <class R> struct Tol;
 template <> struct Tol<float> { static constexpr float out = 1e-6f; };
@@ -258,8 +274,8 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkPar
     const R xi = Unif<R>::closed(rs.next(P));
     if (R(0) < xi && xi < R(1)) {
       const R g2 = g * g;
-      const R t = (R(1) - g2) / (R(1) - g * (R(1) - R(2) * xi));
-      cost = (R(1) + g2 - t * t) / (R(2) * g);
+      const R t = w_div(R(1) - g2, R(1) - g * (R(1) - R(2) * xi));
+      cost = w_div(R(1) + g2 - t * t, R(2) * g);
     } else {
       cost = xi >= R(1) ? R(1) : R(-1);
     }
@@ -269,7 +285,7 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkPar
   cost = fmin(fmax(cost, R(-1)), R(1));
   const R sint = m_sqrt(fmax(R(1) - cost * cost, R(0)));
   R sp, cp;
-  m_sincospi(R(2) * Unif<R>::closed(rs.next(P)), &sp, &cp);
+  w_sincos_rev(Unif<R>::closed(rs.next(P)), &sp, &cp);
   R nx, ny, nz;
   if (fabs(p.d[2]) > R(0.999)) {
     nx = sint * cp;
@@ -277,12 +293,12 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkPar
     nz = p.d[2] >= R(0) ? cost : -cost;
   } else {
     const R tmp = m_sqrt(R(1) - p.d[2] * p.d[2]);
-    const R itmp = R(1) / tmp;
+    const R itmp = w_div(R(1), tmp);
     nx = sint * (p.d[0] * p.d[2] * cp - p.d[1] * sp) * itmp + p.d[0] * cost;
     ny = sint * (p.d[1] * p.d[2] * cp + p.d[0] * sp) * itmp + p.d[1] * cost;
     nz = -sint * cp * tmp + p.d[2] * cost;
   }
-  const R inv = R(1) / m_sqrt(nx * nx + ny * ny + nz * nz);
+  const R inv = w_rsqrt(nx * nx + ny * ny + nz * nz);
   p.d[0] = nx * inv;
   p.d[1] = ny * inv;
   p.d[2] = nz * inv;
@@ -332,10 +348,10 @@ template <class R> __device__ __forceinline__ R gauss_angle(R xi, R sig, R h) {
 // CreatePhoton (MC3D.hpp:1060-1294, MC2D.hpp:958-1132).  Returns the source's
 // mother element and initial face.
 template <int D, class R>
-__device__ __forceinline__ void create_photon(Photon<R> &p, SeqDraws &rs, const WalkParams &P) {
+__device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const WalkParams &P) {
   // source entry: first CDF value above xi, clamped (MC2D.hpp:966-969)
   const double xi = Unif<double>::closed(rs.next(P));
-  int lo = 0, hi = P.nsrc;
+  int lo = 0, hi = P.nsrc > 1 ? P.nsrc : 0;   // one source: nothing to search
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
     if (xi < P.cdf[mid]) hi = mid; else lo = mid + 1;
@@ -624,10 +640,9 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         if (__ballot(invalid)) exhausted = true;
         if (!alive && !invalid) {
           const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
-          SeqDraws cs{plo, phi, 0u, 0u, 0u, 0u};
-          create_photon<D, R>(p, cs, P);
-          p.prop = -m_log(Unif<R>::open(cs.next(P))) / E[p.el].opt.y;
           rs.seed(plo, phi, P);
+          create_photon<D, R>(p, rs, P);
+          p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
           alive = true;
         }
       }
@@ -699,7 +714,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         lose();
         return;
       }
-      const R dist = bn / bc;
+      const R dist = w_div(bn, bc);
       const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
 #pragma unroll
       for (int k = 0; k < D; ++k) p.x[k] += p.d[k] * ds;
@@ -710,7 +725,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       // FP64: the reference's att = exp(-mua ds) and (1 - att) w.  FP32: the
       // deposit -w expm1(-mua ds) is subtracted from w exactly, so the weight
       // ledger closes to rounding of its FP64 sums.
-      const R em = kF64 ? R(0) : m_expm1(-mua * ds);
+      const R em = kF64 ? R(0) : w_expm1_neg(-mua * ds);
       const R att = kF64 ? m_exp(-mua * ds) : R(1) + em;
       const R dabs = kF64 ? p.w - p.w * att : -p.w * em;
       if constexpr (!FD) {
@@ -721,7 +736,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         // phase advance b = k ds taken from half-angle terms
         const R b = R(P.kfac) * nidx * ds;
         R sh, ch;
-        m_sincos(R(0.5) * b, &sh, &ch);
+        w_sincos_small(R(0.5) * b, &sh, &ch);
         const R sb = R(2) * sh * ch, ver = R(2) * sh * sh, cb = R(1) - ver;
         const R c1 = p.cph * cb - p.sph * sb, s1 = p.sph * cb + p.cph * sb;
         R er, ei;
@@ -754,7 +769,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           if constexpr (D == 3) scatter3<R>(p, opt.w, rs, P);
           else scatter2<R>(p, opt.w, rs, P);
         }
-        p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus;
+        p.prop = w_div(w_neglog(Unif<R>::open(rs.next(P))), mus);
         return;
       }
 
@@ -805,7 +820,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       const R mus_c = mus, n_c = nidx;
       if (mus_c <= R(0) && mus_n > R(0)) {
         // vacuum -> scattering medium: fresh free path
-        p.prop = -m_log(Unif<R>::open(rs.next(P))) / mus_n;
+        p.prop = w_neglog(Unif<R>::open(rs.next(P))) / mus_n;
       }
       if (p.w < R(P.weight0)) {
         // Russian roulette at the crossing (MC3D.hpp:1597-1604)
@@ -823,7 +838,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         p.face = bf;
         return;
       }
-      if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= mus_c / mus_n;
+      if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= w_div(mus_c, mus_n);
       if (!back) {
         // the neighbour does not list this element: back-lookup failure
         lose();
