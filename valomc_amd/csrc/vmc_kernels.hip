@@ -491,8 +491,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   void *const btally = reinterpret_cast<char *>(P.btally) + (size_t)rep * P.nb * 4 * 8;
   const int lane = (int)__lane_id();
 
+  constexpr bool kF64 = std::is_same<R, double>::value;
   Photon<R> p;
   WalkRng rs{0u, 0u, 0u, 0u};   // the walk's random stream of the lane's photon
+  uint32_t nrecheck = 0;        // FP32 exit-face choices decided in FP64
   bool alive = false;
   // absorbed weight accumulates per lane; the rare end-of-photon sums go to
   // per-wave LDS slots [exited, killed, gained, lost, loss count]
@@ -681,8 +683,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       };
       // exit face: smallest positive plane distance over faces moving outward,
       // the current face excluded (MC3D.hpp:1020-1056)
-      R bn = R(1), bc = R(0);
-      int bf = -1, code = kNbNone;
+      R bn = R(1), bc = R(0), sn = R(1), sc = R(0);   // best and runner-up distance bn/bc, sn/sc
+      int bf = -1, sf = -1, code = kNbNone;
       R nrm[3] = {R(0), R(0), R(0)};
       R mus_n = R(0), n_n = R(0);
       bool out_cur = false;
@@ -697,22 +699,76 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         const R num = fmax(pl[f].w - q, R(0));
         const bool cur = f == p.face;
         out_cur = cur ? c > Tol<R>::out : out_cur;
-        if (!cur && c > R(0) && num * bc < bn * c) {
-          bn = num;
-          bc = c;
-          bf = f;
-          code = nb[f];
-          nrm[0] = pl[f].x;
-          nrm[1] = pl[f].y;
-          nrm[2] = D == 3 ? pl[f].z : R(0);
-          mus_n = nbo[f].x;
-          n_n = nbo[f].y;
+        if (!cur && c > R(0)) {
+          if (num * bc < bn * c) {
+            if constexpr (!kF64) {
+              sn = bn;
+              sc = bc;
+              sf = bf;
+            }
+            bn = num;
+            bc = c;
+            bf = f;
+            code = nb[f];
+            nrm[0] = pl[f].x;
+            nrm[1] = pl[f].y;
+            nrm[2] = D == 3 ? pl[f].z : R(0);
+            mus_n = nbo[f].x;
+            n_n = nbo[f].y;
+          } else if (!kF64 && num * sc < sn * c) {
+            sn = num;
+            sc = c;
+            sf = f;
+          }
         }
       }
       if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
         // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
         lose();
         return;
+      }
+      if constexpr (!kF64) {
+        // FP32 exit distances within 2^-16 of each other: decide in FP64
+        // from the same record and packet (the record is re-read from the
+        // cache rather than kept live through the loop)
+        if (sf >= 0 && sn * bc - bn * sc < R(1.52587890625e-5) * sn * bc) {
+          ++nrecheck;
+          V4 pl2[NF];
+          V4 opt2;
+          i4 nb2;
+          typename Vec2T<R>::type nbo2[NF];
+          load_elem<NF, R>(E, p.el, pl2, opt2, nb2, nbo2);
+          auto t64 = [&](int f) {
+            double c = 0, q = 0;
+#pragma unroll
+            for (int g = 0; g < NF; ++g) {
+              if (g != f) continue;
+              c = (double)pl2[g].x * p.d[0] + (double)pl2[g].y * p.d[1];
+              q = (double)pl2[g].x * p.x[0] + (double)pl2[g].y * p.x[1];
+              if (D == 3) {
+                c += (double)pl2[g].z * p.d[2];
+                q += (double)pl2[g].z * p.x[2];
+              }
+              q = fmax((double)pl2[g].w - q, 0.0);
+            }
+            return q / c;
+          };
+          if (t64(sf) < t64(bf)) {
+            bn = sn;
+            bc = sc;
+            bf = sf;
+#pragma unroll
+            for (int g = 0; g < NF; ++g) {
+              if (g != sf) continue;
+              code = nb2[g];
+              nrm[0] = pl2[g].x;
+              nrm[1] = pl2[g].y;
+              nrm[2] = D == 3 ? pl2[g].z : R(0);
+              mus_n = nbo2[g].x;
+              n_n = nbo2[g].y;
+            }
+          }
+        }
       }
       const R dist = w_div(bn, bc);
       const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
@@ -721,7 +777,6 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
       const R mua = opt.x, mus = opt.y, nidx = opt.z;
       // element tally and attenuation (MC3D.hpp:1479-1526)
-      constexpr bool kF64 = std::is_same<R, double>::value;
       // FP64: the reference's att = exp(-mua ds) and (1 - att) w.  FP32: the
       // deposit -w expm1(-mua ds) is subtracted from w exactly, so the weight
       // ledger closes to rounding of its FP64 sums.
@@ -882,7 +937,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
   // ---- flush the counters: one set of global atomics per wave ----
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) l_abs += __shfl_xor(l_abs, off);
+  for (int off = 32; off > 0; off >>= 1) {
+    l_abs += __shfl_xor(l_abs, off);
+    nrecheck += __shfl_xor(nrecheck, off);
+  }
   __syncthreads();
   // ---- flush the sector cache into this workgroup's replica: one sector
   // per WS consecutive lanes ----
@@ -899,6 +957,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     unsafeAtomicAdd(&P.ledger[4], s_led[wid][3]);
     atomicAdd(&P.counters[1], (unsigned long long)steps);
     atomicAdd(&P.counters[2], (unsigned long long)s_led[wid][4]);
+    if (nrecheck) atomicAdd(&P.counters[3], (unsigned long long)nrecheck);
   }
 }
 
