@@ -141,6 +141,28 @@ int vmc_problem_set_medium(vmc_problem *prob, const double *mua, const double *m
 
 void vmc_problem_destroy(vmc_problem *prob);
 
+/* One instance of a batch: a resident problem, its run and its outputs. */
+typedef struct vmc_batch_item {
+  vmc_problem *prob;
+  vmc_run_desc run;      /* Nphoton, seed, stream_tag, f, phase0 per item;
+                            precision and tally mode must agree across the
+                            batch, the tuning fields of item 0 apply to all */
+  vmc_output out;        /* normalised like vmc_problem_run                 */
+  int64_t loss;          /* out: MC3D::loss of this item                     */
+  int32_t status;        /* out                                              */
+  int32_t reserved;
+} vmc_batch_item;
+
+/* Many forward solves in one persistent launch per (dimension, CW/FD)
+   group: the photons of all items share the GPU, each item keeps its own
+   random streams (seed, stream_tag, photon index) and its own tallies, so
+   every result equals the one vmc_problem_run gives for that item alone (bit
+   for bit in VMC_TALLY_FIXED64 mode).  No counterpart in the reference, which
+   runs one MonteCarlo() per object; this is the batched path the inverse
+   solvers that call MC3Dmex in a loop (ValoMC.m) would use.  stats (optional)
+   sums the whole batch. */
+int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats);
+
 /* One-shot call with the argument list of the MEX gateways
    (cpp/3d/MC3Dmex.cpp:106-110, cpp/2d/MC2Dmex.cpp:105-109): create, run,
    destroy.  HN may be NULL; HN_out (optional) receives the table used.

@@ -1,0 +1,98 @@
+"""Batched forward solves (vmc_batch_run) against single runs.
+
+Every item keeps its own random streams (seed, stream tag, photon index)
+and its own tallies, so in fixed-point mode each item's result in a mixed
+batch equals its result alone bit for bit; FP64 tallies agree to rounding.
+The batch mixes problems of both dimensions, CW and frequency-domain items,
+and uneven photon counts (including one not a multiple of the dispenser's
+chunk and an empty item).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+OUT = ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")
+
+
+@pytest.fixture(scope="module")
+def vmc():
+    import valomc_amd
+    return valomc_amd
+
+
+def _golden(name):
+    return cases.from_arrays(dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)))
+
+
+MIX = [("p3_pencil_sphere", dict(nphoton=30000, seed=3)),
+       ("p3_pencil_sphere", dict(nphoton=12345, seed=3, stream_tag=7)),
+       ("p3_cosinic_hetero_n", dict(nphoton=20000, seed=11)),
+       ("p3_direct_abs_fd", dict(nphoton=15000, seed=5)),
+       ("p2_square_cosinic", dict(nphoton=25000, seed=2)),
+       ("p2_iso_gauss_fd", dict(nphoton=10000, seed=9)),
+       ("p3_iso_mirror", dict(nphoton=0, seed=1))]
+
+
+def _mix(vmc):
+    probs = {}
+    items = []
+    for name, kw in MIX:
+        p = _golden(name)
+        if name not in probs:
+            probs[name] = (p, vmc.Problem(p))
+        pp, prob = probs[name]
+        items.append((prob, dict(kw, f=pp.f, phase0=pp.phase0)))
+    return probs, items
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_items_equal_single_runs(vmc, precision):
+    prec = vmc.FP32 if precision == "fp32" else vmc.FP64
+    probs, items = _mix(vmc)
+    try:
+        print(f"batch of {len(items)} items, {precision}, fixed-point tallies", flush=True)
+        res, stats = vmc.batch_run(items, precision=prec, tally=vmc.TALLY_FIXED64)
+        assert stats.photons == sum(kw["nphoton"] for _, kw in items)
+        for (prob, kw), r in zip(items, res):
+            kw = dict(kw)
+            alone = prob.run(kw.pop("nphoton"), precision=prec, tally=vmc.TALLY_FIXED64, **kw)
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
+            assert r.loss == alone.loss
+    finally:
+        for _, prob in probs.values():
+            prob.close()
+
+
+def test_batch_fp64_tallies_match_single_runs(vmc):
+    probs, items = _mix(vmc)
+    try:
+        res, _ = vmc.batch_run(items, precision=vmc.FP64)
+        for (prob, kw), r in zip(items, res):
+            kw = dict(kw)
+            n = kw.pop("nphoton")
+            alone = prob.run(n, precision=vmc.FP64, **kw)
+            for k in OUT:
+                a, b = getattr(r, k), getattr(alone, k)
+                if n == 0:
+                    assert np.array_equal(a, b, equal_nan=True), k   # 0/0 like the reference
+                    continue
+                nb = np.linalg.norm(b)
+                assert np.linalg.norm(a - b) <= 1e-12 * max(nb, 1e-300), (kw, k)
+    finally:
+        for _, prob in probs.values():
+            prob.close()
+
+
+def test_batch_rejects_mixed_precision(vmc):
+    p = _golden("p3_pencil_sphere")
+    with vmc.Problem(p) as prob:
+        items = [(prob, dict(nphoton=100, precision=vmc.FP32)), (prob, dict(nphoton=100, precision=vmc.FP64))]
+        with pytest.raises(vmc.VmcError):
+            vmc.batch_run(items)

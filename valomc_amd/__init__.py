@@ -104,6 +104,7 @@ def load_library():
         lib.vmc_problem_get_hn.argtypes = [C.c_void_p, C.c_void_p]
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
+        lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
         lib.vmc_problem_destroy.restype = None
         lib.vmc_status_string.restype = C.c_char_p
         lib.vmc_last_error.restype = C.c_char_p
@@ -152,6 +153,25 @@ class RunResult:
     block_size: int
     status: int = 0
     rechecks: int = 0
+
+
+def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
+                  precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
+                  blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
+                  lds_slots: int = 0, queue_len: int = 0) -> RunDesc:
+    """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
+    rd = RunDesc()
+    rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
+    rd.precision, rd.tally = int(precision), int(tally)
+    rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
+    rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
+    rd.queue_len = int(queue_len)
+    return rd
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("prob", C.c_void_p), ("run", RunDesc), ("out", Output), ("loss", C.c_int64),
+                ("status", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Problem:
@@ -204,6 +224,12 @@ class Problem:
     def __exit__(self, *exc):
         self.close()
 
+    def _outputs(self):
+        out = dict(ER=np.zeros(self.Ne), EI=np.zeros(self.Ne), EBR=np.zeros(self.Nb),
+                   EBI=np.zeros(self.Nb), DEBR=np.zeros(self.Nb), DEBI=np.zeros(self.Nb))
+        o = Output(*[out[k].ctypes.data for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")])
+        return out, o
+
     def hn(self) -> np.ndarray:
         nv = self.dim + 1
         out = np.zeros(self.Ne * nv, dtype=np.int64)
@@ -223,15 +249,11 @@ class Problem:
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
             lds_slots: int = 0, queue_len: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
-        rd = RunDesc()
-        rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
-        rd.precision, rd.tally = int(precision), int(tally)
-        rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
-        rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
-        rd.queue_len = int(queue_len)
-        out = dict(ER=np.zeros(self.Ne), EI=np.zeros(self.Ne), EBR=np.zeros(self.Nb),
-                   EBI=np.zeros(self.Nb), DEBR=np.zeros(self.Nb), DEBI=np.zeros(self.Nb))
-        o = Output(*[out[k].ctypes.data for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI")])
+        rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
+                           block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
+                           refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
+                           queue_len=queue_len)
+        out, o = self._outputs()
         s = Stats()
         cb = None
         if progress is not None:
@@ -244,6 +266,36 @@ class Problem:
                          total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)),
                          photons=int(s.photons), grid_blocks=int(s.grid_blocks),
                          block_size=int(s.block_size), rechecks=int(s.rechecks), **out)
+
+
+def batch_run(items, **common):
+    """vmc_batch_run: ``items`` is a list of ``(Problem, run_kwargs)``; keyword
+    arguments in ``common`` apply to every item (precision, tally, tuning).
+    Returns one RunResult per item (``steps``, ``kernel_ms`` and the ledger
+    are the batch's) and the batch RunResult-style stats."""
+    lib = load_library()
+    arr = (BatchItem * len(items))()
+    outs = []
+    for k, (prob, kw) in enumerate(items):
+        kw = {**common, **kw}
+        n = kw.pop("nphoton")
+        arr[k].prob = prob._h
+        arr[k].run = make_run_desc(n, **kw)
+        out, o = prob._outputs()
+        arr[k].out = o
+        outs.append(out)
+    s = Stats()
+    st = lib.vmc_batch_run(arr, len(items), C.byref(s))
+    if st:
+        raise VmcError(st, lib)
+    res = []
+    for k, out in enumerate(outs):
+        res.append(RunResult(status=int(arr[k].status), loss=int(arr[k].loss), steps=int(s.steps),
+                             kernel_ms=float(s.kernel_ms), total_ms=float(s.total_ms),
+                             ledger=np.array(list(s.ledger)), photons=int(arr[k].run.Nphoton),
+                             grid_blocks=int(s.grid_blocks), block_size=int(s.block_size),
+                             rechecks=int(s.rechecks), **out))
+    return res, s
 
 
 class _EngineBase:

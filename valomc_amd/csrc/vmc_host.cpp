@@ -18,13 +18,14 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
 namespace vmc {
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, size_t lds_bytes,
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, int block, size_t lds_bytes,
                           int *blocks_per_cu);
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, size_t lds_bytes,
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, int grid, int block, size_t lds_bytes,
                        hipStream_t s, const WalkParams &P);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, double inv_scale,
@@ -76,6 +77,40 @@ template <class T> struct DevBuf {
   }
 };
 
+// Tally arena and launch resources of one stream: nrep replicas of
+// [ne][W] element words and [nb][4] boundary words, the launch's item
+// records, the weight ledger, the counters and a per-item loss count.
+struct Arena {
+  int64_t ne = 0, nb = 0;
+  int W = 1, nrep = 0;
+  size_t nitems = 0;
+  DevBuf<unsigned long long> etally, btally, counters, item_loss;
+  DevBuf<double> ledger, raw_e, raw_b;
+  DevBuf<ItemRec> items;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void open() {
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&ev1), "hipEventCreate");
+  }
+  void close() {
+    etally.release();
+    btally.release();
+    counters.release();
+    item_loss.release();
+    ledger.release();
+    raw_e.release();
+    raw_b.release();
+    items.release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+    ev0 = ev1 = nullptr;
+    stream = nullptr;
+  }
+};
+
 } // namespace
 
 struct vmc_problem {
@@ -96,14 +131,11 @@ struct vmc_problem {
 
   int device = 0, num_cus = 0;
   int64_t lds_per_cu = 65536;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf<unsigned char> elems32, elems64;
   DevBuf<double> cdf;
   DevBuf<SrcRec> dsrc;
   DevBuf<BndRec> dbnd;
-  DevBuf<unsigned long long> etally, btally, counters;
-  DevBuf<double> ledger, raw_e, raw_b;
+  Arena arena;   // tallies and stream of this problem's own runs
   bool params_dirty32 = true, params_dirty64 = true;
 
   int64_t h(int64_t el, int c) const { return H[el + Ne * c]; }
@@ -673,10 +705,6 @@ template <class T> void copy_in(std::vector<T> &dst, const T *src, int64_t n) {
 
 } // namespace
 
-extern "C" {
-
-} // extern "C"
-
 namespace {
 
 // ErrorChecks() and the host part of Init(); returns a status.
@@ -737,6 +765,221 @@ int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
 
 } // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// Launch engine: persistent walk launches over a list of items (problem
+// instances), shared by single runs and batches
+// ---------------------------------------------------------------------------
+
+const uint64_t kChunk = 128;   // photon ids a wave takes from the dispenser at once
+
+struct LaunchCfg {
+  int dim = 3;
+  bool f64 = false, fd = false, fixed = false, xcc = true;
+  int W = 1, block = 256, qlen = 0, slots = 0, refill = 6, nrep = 8;
+  size_t lds = 0;
+};
+
+// Validates a run's precision, tally and tuning fields and resolves the
+// launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of LDS.
+int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c) {
+  (void)num_cus;
+  if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
+  if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
+  c->dim = dim;
+  c->f64 = run->precision == VMC_FP64;
+  c->fixed = run->tally == VMC_TALLY_FIXED64;
+  c->fd = fd;
+  c->W = fd ? 2 : 1;
+  // FP32 transport runs as one 1024-lane workgroup per CU (16 waves sharing
+  // one sector cache), FP64 (occupancy 2 waves/SIMD) as 256-lane
+  // workgroups, two per CU
+  c->block = run->block_size > 0 ? run->block_size : (c->f64 ? 256 : 1024);
+  if (c->block % 64 != 0 || c->block > (c->f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
+  // replicas == 0: one per XCD, added with L2 atomics; > 0: that many,
+  // picked by workgroup index and added with device-scope atomics
+  c->xcc = run->replicas <= 0;
+  c->nrep = c->xcc ? 8 : run->replicas;
+  if (c->nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
+  // run queue (sectors per wave): 0 = default, -1 = off (direct atomics)
+  const int rbytes = c->f64 ? 8 : 4;
+  c->qlen = run->queue_len == 0 ? (fd ? 32 : 64) : (run->queue_len < 0 ? 0 : run->queue_len);
+  if (c->qlen > 1024) return VMC_ERR_INVALID_ARGUMENT;
+  // sector cache: 0 = fill the CU's LDS left over by the queues and lane
+  // buffers of the workgroups that fit by registers, -1 = off
+  c->slots = run->lds_slots < 0 ? 0 : run->lds_slots;
+  if (run->lds_slots == 0) {
+    const int waves_per_cu = c->f64 ? 8 : 16;
+    const int bpc_target = std::max(1, waves_per_cu / (c->block / 64));
+    const size_t per_block = (size_t)lds_per_cu / bpc_target;
+    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->qlen, c->block).total + 1024;   // + ledger, alignment
+    const size_t entry = (size_t)kSectorElems * c->W * 8 + 4;
+    c->slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
+  }
+  if (c->slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
+  c->lds = WalkLds(c->W, rbytes, c->slots, c->qlen, c->block).total;
+  // creating photons for several dead lanes at once keeps the creation
+  // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
+  c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
+  return VMC_OK;
+}
+
+// Sizes the arena for ne element and nb boundary slots and clears its
+// tallies, ledger, counters and per-item loss counts.
+void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems) {
+  a.ne = ne;
+  a.nb = nb;
+  a.W = c.W;
+  a.nrep = c.nrep;
+  a.nitems = nitems;
+  const size_t ew = (size_t)ne * c.W * c.nrep, bw = (size_t)nb * 4 * c.nrep;
+  a.etally.ensure(ew);
+  a.btally.ensure(bw);
+  a.raw_e.ensure((size_t)ne * c.W);
+  a.raw_b.ensure((size_t)nb * 4);
+  a.ledger.ensure(8);
+  a.counters.ensure(8);
+  a.item_loss.ensure(nitems);
+  a.items.ensure(nitems);
+  hip_check(launch_fill_u64(a.etally.p, ew, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.btally.p, bw, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), 8, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.counters.p, 8, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.item_loss.p, nitems, 0ull, a.stream), "fill");
+}
+
+// One persistent launch over `items` (vstart set here, chunk-aligned and in
+// order).  Returns the kernel time from HIP events on the arena's stream and
+// the grid size.
+float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out) {
+  uint64_t v = 0;
+  for (ItemRec &it : items) {
+    it.vstart = v;
+    v += (it.count + kChunk - 1) / kChunk * kChunk;
+  }
+  if (v == 0) return 0.f;
+  const bool batch = items.size() > 1;
+  int bpc = 0;
+  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, batch, c.block, c.lds, &bpc), "occupancy");
+  if (bpc < 1) throw HipFail{hipErrorInvalidConfiguration, "workgroup does not fit on a CU (block size / LDS tally cache)"};
+  const int grid = bpc * num_cus;
+  hip_check(hipMemcpyAsync(a.items.p, items.data(), items.size() * sizeof(ItemRec), hipMemcpyHostToDevice, a.stream),
+            "upload");
+  hip_check(launch_fill_u64(a.counters.p, 1, 0ull, a.stream), "fill");   // dispenser
+  WalkParams P{};
+  P.items = a.items.p;
+  P.nitems = (int32_t)items.size();
+  P.vtotal = v;
+  P.etally = a.etally.p;
+  P.btally = a.btally.p;
+  P.ledger = a.ledger.p;
+  P.counters = a.counters.p;
+  P.item_loss = a.item_loss.p;
+  P.ne = a.ne;
+  P.nb = a.nb;
+  P.nrep = c.nrep;
+  P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
+  P.refill_min = c.refill;
+  P.chunk = (int32_t)kChunk;
+  P.xcc_replicas = c.xcc ? 1 : 0;
+  P.lds_slots = c.slots;
+  P.queue_len = c.qlen;
+  P.weight0 = 0.001;   // MC3D.hpp:182
+  P.chance = 0.1;      // MC3D.hpp:183
+  P.fix_scale = 4294967296.0;
+  hip_check(hipEventRecord(a.ev0, a.stream), "event");
+  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, batch, grid, c.block, c.lds, a.stream, P), "walk");
+  hip_check(hipEventRecord(a.ev1, a.stream), "event");
+  hip_check(hipEventSynchronize(a.ev1), "sync");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a.ev0, a.ev1), "event");
+  if (grid_out) *grid_out = grid;
+  return ms;
+}
+
+// Sums the replicas and copies raw element words [Ne_pad*W], boundary
+// words [nb*4], the ledger, the counters and the per-item loss counts to
+// the host.
+void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
+                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+  const double inv = c.fixed ? 1.0 / 4294967296.0 : 1.0;
+  const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
+  raw_e.resize(ew);
+  raw_b.resize(bw);
+  loss.resize(a.nitems);
+  hip_check(launch_reduce(c.fixed, a.etally.p, a.raw_e.p, ew, c.nrep, inv, a.stream), "reduce");
+  hip_check(launch_reduce(c.fixed, a.btally.p, a.raw_b.p, bw, c.nrep, inv, a.stream), "reduce");
+  hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, ew * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, bw * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(led, a.ledger.p, 5 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(cnt, a.counters.p, 4 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  if (a.nitems)
+    hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, a.nitems * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipStreamSynchronize(a.stream), "sync");
+}
+
+// Item record of a problem run: its device arrays, stream key and phase.
+ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint64_t count, uint64_t id0,
+                  int64_t tbase, int64_t bbase) {
+  ItemRec it{};
+  it.count = count;
+  it.id0 = id0;
+  it.elems = f64 ? (const void *)m.elems64.p : (const void *)m.elems32.p;
+  it.cdf = m.cdf.p;
+  it.src = m.dsrc.p;
+  it.bnd = m.dbnd.p;
+  it.tbase = (int32_t)tbase;
+  it.bbase = (int32_t)bbase;
+  it.nsrc = (int32_t)m.src.size();
+  it.key0 = (uint32_t)run->seed;
+  it.key1 = (uint32_t)(run->seed >> 32);
+  it.tag = run->stream_tag;
+  double phase0 = run->phase0;
+  if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
+  it.kfac = 2.0 * M_PI * run->f / kC0;
+  it.cph0 = cos(phase0);
+  it.sph0 = sin(phase0);
+  return it;
+}
+
+// Splits a problem's raw element words (device order, from slot tbase) and
+// boundary words (from slot bbase) and writes normalised outputs
+// (MC3D.hpp:1793-1818).
+void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
+                    vmc_output *out) {
+  const double omega = 2.0 * M_PI * f;
+  const bool fd = omega > 0.0;
+  std::vector<double> ER(m.Ne), EI(m.Ne, 0.0), EBR(m.Nb), EBI(m.Nb), DEBR(m.Nb), DEBI(m.Nb), kn(m.Ne);
+  for (int64_t i = 0; i < m.Ne; ++i) {
+    const int64_t dv = m.inv[i];
+    ER[i] = raw_e[W * dv];
+    if (fd) EI[i] = raw_e[W * dv + 1];
+    kn[i] = omega / kC0 * m.n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+  }
+  for (int64_t i = 0; i < m.Nb; ++i) {
+    EBR[i] = raw_b[4 * i];
+    EBI[i] = raw_b[4 * i + 1];
+    DEBR[i] = raw_b[4 * i + 2];
+    DEBI[i] = raw_b[4 * i + 3];
+  }
+  normalise(m, omega, kn.data(), N, ER.data(), EI.data(), EBR.data(), EBI.data(), DEBR.data(), DEBI.data());
+  if (!out) return;
+  if (out->ER) std::memcpy(out->ER, ER.data(), 8 * m.Ne);
+  if (out->EI) std::memcpy(out->EI, EI.data(), 8 * m.Ne);
+  if (out->EBR) std::memcpy(out->EBR, EBR.data(), 8 * m.Nb);
+  if (out->EBI) std::memcpy(out->EBI, EBI.data(), 8 * m.Nb);
+  if (out->DEBR) std::memcpy(out->DEBR, DEBR.data(), 8 * m.Nb);
+  if (out->DEBI) std::memcpy(out->DEBI, DEBI.data(), 8 * m.Nb);
+}
+
+int64_t sector_round(int64_t n) { return (n + kSectorElems - 1) / kSectorElems * kSectorElems; }
+
+Arena g_batch_arena;   // tallies of vmc_batch_run (one batch at a time)
+std::mutex g_batch_mutex;
+
+} // namespace
+
 extern "C" {
 
 int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsources) {
@@ -771,17 +1014,13 @@ int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
     m->num_cus = prop.multiProcessorCount;
     m->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
                                                              : (int64_t)prop.sharedMemPerBlock;
-    hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate");
-    hip_check(hipEventCreate(&m->ev0), "hipEventCreate");
-    hip_check(hipEventCreate(&m->ev1), "hipEventCreate");
+    m->arena.open();
     m->cdf.ensure(m->lsCDF.size());
     hip_check(hipMemcpy(m->cdf.p, m->lsCDF.data(), m->lsCDF.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
     m->dsrc.ensure(m->src.size());
     hip_check(hipMemcpy(m->dsrc.p, m->src.data(), m->src.size() * sizeof(SrcRec), hipMemcpyHostToDevice), "upload");
     m->dbnd.ensure(m->bnd.size());
     hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
-    m->ledger.ensure(8);
-    m->counters.ensure(8);
   } catch (HipFail f) {
     int st = fail_hip(f);
     vmc_problem_destroy(m);
@@ -811,159 +1050,42 @@ int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) {
 int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
                     vmc_progress_fn progress, void *user) {
   if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
-  if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
-  if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
   const auto t_start = std::chrono::steady_clock::now();
-  const bool f64 = run->precision == VMC_FP64;
-  const bool fixed = run->tally == VMC_TALLY_FIXED64;
-  const double omega = 2.0 * M_PI * run->f;
-  const bool fd = omega > 0.0;
-  double phase0 = run->phase0;
-  if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
-  // Launch shape: FP32 transport runs as one 1024-lane workgroup per CU (16
-  // waves sharing one sector cache), FP64 (occupancy 2 waves/SIMD) as
-  // 256-lane workgroups, two per CU.
-  int block = run->block_size > 0 ? run->block_size : (f64 ? 256 : 1024);
-  if (block % 64 != 0 || block > (f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
-  // replicas == 0: one per XCD, added with L2 atomics; > 0: that many,
-  // picked by workgroup index and added with device-scope atomics
-  const bool xcc = run->replicas <= 0;
-  int nrep = xcc ? 8 : run->replicas;
-  if (nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
-  // run queue (sectors per wave): 0 = default, -1 = off (direct atomics)
-  const int W = fd ? 2 : 1, rbytes = f64 ? 8 : 4;
-  const int qlen = run->queue_len == 0 ? (fd ? 32 : 64) : (run->queue_len < 0 ? 0 : run->queue_len);
-  if (qlen > 1024) return VMC_ERR_INVALID_ARGUMENT;
-  // sector cache: 0 = fill the CU's LDS left over by the queues and lane
-  // buffers of the workgroups that fit by registers, -1 = off
-  int slots = run->lds_slots < 0 ? 0 : run->lds_slots;
-  if (run->lds_slots == 0) {
-    const int waves_per_cu = f64 ? 8 : 16;
-    const int bpc_target = std::max(1, waves_per_cu / (block / 64));
-    const size_t per_block = (size_t)m->lds_per_cu / bpc_target;
-    const size_t fixed_part = WalkLds(W, rbytes, 0, qlen, block).total + 1024;   // + ledger slots, alignment
-    const size_t entry = (size_t)kSectorElems * W * 8 + 4;
-    slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
-  }
-  if (slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
-  const size_t lds_bytes = WalkLds(W, rbytes, slots, qlen, block).total;
+  const bool fd = 2.0 * M_PI * run->f > 0.0;
+  LaunchCfg c;
+  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c);
+  if (st != VMC_OK) return st;
   const int64_t N = run->Nphoton;
-  // element tally words per replica: Ne rounded up to whole sectors
-  const int64_t ne_pad = (m->Ne + kSectorElems - 1) / kSectorElems * kSectorElems;
-  const size_t ne_slots = (size_t)ne_pad * W, nb_slots = (size_t)m->Nb * 4;
-  int aborted = 0;
+  int aborted = 0, grid = 0;
   float kernel_ms = 0.f;
-  int grid = 0;
   double led[5] = {0, 0, 0, 0, 0};
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  std::vector<double> raw_e(ne_slots), raw_b(nb_slots);
+  std::vector<double> raw_e, raw_b;
+  std::vector<unsigned long long> loss;
   try {
     hip_check(hipSetDevice(m->device), "hipSetDevice");
-    upload_params(*m, f64);
-    int bpc = 0;
-    hip_check(walk_occupancy(m->dim, f64, fd, fixed, block, lds_bytes, &bpc), "occupancy");
-    if (bpc < 1) {
-      g_last_error = "workgroup does not fit on a CU (block size / LDS tally cache too large)";
-      return VMC_ERR_INVALID_ARGUMENT;
-    }
-    if (run->blocks_per_cu > 0) bpc = std::min(bpc, run->blocks_per_cu);
-    grid = bpc * m->num_cus;
-    m->etally.ensure(ne_slots * nrep);
-    m->btally.ensure(nb_slots * nrep);
-    m->raw_e.ensure(ne_slots);
-    m->raw_b.ensure(nb_slots);
-    hip_check(launch_fill_u64(m->etally.p, ne_slots * nrep, 0ull, m->stream), "fill");
-    hip_check(launch_fill_u64(m->btally.p, nb_slots * nrep, 0ull, m->stream), "fill");
-    hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(m->ledger.p), 8, 0ull, m->stream), "fill");
-    hip_check(launch_fill_u64(m->counters.p, 8, 0ull, m->stream), "fill");
-
-    WalkParams P{};
-    P.elems = f64 ? (const void *)m->elems64.p : (const void *)m->elems32.p;
-    P.cdf = m->cdf.p;
-    P.src = m->dsrc.p;
-    P.bnd = m->dbnd.p;
-    P.etally = m->etally.p;
-    P.btally = m->btally.p;
-    P.ledger = m->ledger.p;
-    P.counters = m->counters.p;
-    P.ne = ne_pad;
-    P.nb = m->Nb;
-    P.nsrc = (int32_t)m->src.size();
-    P.nrep = nrep;
-    P.key0 = (uint32_t)run->seed;
-    P.key1 = (uint32_t)(run->seed >> 32);
-    P.tag = run->stream_tag;
-    P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
-    // creating photons for several dead lanes at once keeps the creation
-    // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
-    P.refill_min = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
-    P.chunk = 128;
-    P.xcc_replicas = xcc ? 1 : 0;
-    P.lds_slots = slots;
-    P.queue_len = qlen;
-    P.phase0 = phase0;
-    P.cph0 = cos(phase0);
-    P.sph0 = sin(phase0);
-    P.kfac = omega / kC0;
-    P.weight0 = 0.001;   // MC3D.hpp:182
-    P.chance = 0.1;      // MC3D.hpp:183
-    P.fix_scale = 4294967296.0;
-
+    upload_params(*m, c.f64);
+    arena_reset(m->arena, c, sector_round(m->Ne), m->Nb, 1);
     // photon ranges: one launch, or ~20 when a progress callback wants ticks
     const int nseg = progress ? 20 : 1;
     int64_t done = 0;
     for (int sgm = 0; sgm < nseg && done < N; ++sgm) {
       const int64_t end = (sgm == nseg - 1) ? N : (N * (sgm + 1)) / nseg;
       if (end <= done) continue;
-      hip_check(launch_fill_u64(m->counters.p, 1, (unsigned long long)done, m->stream), "fill");
-      P.n_end = (uint64_t)end;
-      hip_check(hipEventRecord(m->ev0, m->stream), "event");
-      hip_check(launch_walk(m->dim, f64, fd, fixed, grid, block, lds_bytes, m->stream, P), "walk");
-      hip_check(hipEventRecord(m->ev1, m->stream), "event");
-      hip_check(hipEventSynchronize(m->ev1), "sync");
-      float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, m->ev0, m->ev1), "event");
-      kernel_ms += ms;
+      std::vector<ItemRec> items{make_item(*m, run, c.f64, (uint64_t)(end - done), (uint64_t)done, 0, 0)};
+      kernel_ms += launch_items(m->arena, m->num_cus, c, items, &grid);
       done = end;
       if (progress && !progress(100.0 * (double)done / (double)(N > 0 ? N : 1), user)) {
         aborted = 1;
         break;
       }
     }
-    const double inv = fixed ? 1.0 / 4294967296.0 : 1.0;
-    hip_check(launch_reduce(fixed, m->etally.p, m->raw_e.p, ne_slots, nrep, inv, m->stream), "reduce");
-    hip_check(launch_reduce(fixed, m->btally.p, m->raw_b.p, nb_slots, nrep, inv, m->stream), "reduce");
-    hip_check(hipMemcpyAsync(raw_e.data(), m->raw_e.p, ne_slots * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
-    hip_check(hipMemcpyAsync(raw_b.data(), m->raw_b.p, nb_slots * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
-    hip_check(hipMemcpyAsync(led, m->ledger.p, 5 * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
-    hip_check(hipMemcpyAsync(cnt, m->counters.p, 4 * 8, hipMemcpyDeviceToHost, m->stream), "d2h");
-    hip_check(hipStreamSynchronize(m->stream), "sync");
+    arena_collect(m->arena, c, raw_e, raw_b, led, cnt, loss);
   } catch (HipFail f) {
     return fail_hip(f);
   }
-  // split raw tallies and normalise with the reference's expressions
-  std::vector<double> ER(m->Ne), EI(m->Ne, 0.0), EBR(m->Nb), EBI(m->Nb), DEBR(m->Nb), DEBI(m->Nb), kn(m->Ne);
-  for (int64_t i = 0; i < m->Ne; ++i) {
-    const int64_t dv = m->inv[i];
-    ER[i] = fd ? raw_e[2 * dv] : raw_e[dv];
-    if (fd) EI[i] = raw_e[2 * dv + 1];
-    kn[i] = omega / kC0 * m->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
-  }
-  for (int64_t i = 0; i < m->Nb; ++i) {
-    EBR[i] = raw_b[4 * i];
-    EBI[i] = raw_b[4 * i + 1];
-    DEBR[i] = raw_b[4 * i + 2];
-    DEBI[i] = raw_b[4 * i + 3];
-  }
-  normalise(*m, omega, kn.data(), N, ER.data(), EI.data(), EBR.data(), EBI.data(), DEBR.data(), DEBI.data());
-  if (out) {
-    if (out->ER) std::memcpy(out->ER, ER.data(), 8 * m->Ne);
-    if (out->EI) std::memcpy(out->EI, EI.data(), 8 * m->Ne);
-    if (out->EBR) std::memcpy(out->EBR, EBR.data(), 8 * m->Nb);
-    if (out->EBI) std::memcpy(out->EBI, EBI.data(), 8 * m->Nb);
-    if (out->DEBR) std::memcpy(out->DEBR, DEBR.data(), 8 * m->Nb);
-    if (out->DEBI) std::memcpy(out->DEBI, DEBI.data(), 8 * m->Nb);
-  }
+  // like the reference, an aborted run still divides by the planned N
+  finish_outputs(*m, run->f, N, c.W, raw_e.data(), raw_b.data(), out);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->photons = N;
@@ -973,11 +1095,92 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
     stats->kernel_ms = kernel_ms;
     for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
     stats->grid_blocks = grid;
-    stats->block_size = block;
+    stats->block_size = c.block;
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
   return aborted ? VMC_ERR_ABORTED : VMC_OK;
+}
+
+int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
+  if (!items || n <= 0) return VMC_ERR_INVALID_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  vmc_problem *m0 = items[0].prob;
+  if (!m0) return VMC_ERR_INVALID_ARGUMENT;
+  // one launch per (dimension, CW/FD) group; precision, tally mode and
+  // tuning come from the first item and must agree across the batch
+  for (int32_t i = 0; i < n; ++i) {
+    const vmc_batch_item &b = items[i];
+    if (!b.prob || b.run.Nphoton < 0 || b.prob->device != m0->device) return VMC_ERR_INVALID_ARGUMENT;
+    if (b.run.precision != items[0].run.precision || b.run.tally != items[0].run.tally)
+      return VMC_ERR_INVALID_ARGUMENT;
+    items[i].loss = 0;
+    items[i].status = VMC_OK;
+  }
+  vmc_stats tot{};
+  float kernel_ms = 0.f;
+  try {
+    hip_check(hipSetDevice(m0->device), "hipSetDevice");
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    if (!g_batch_arena.stream) g_batch_arena.open();
+    for (int dim = 2; dim <= 3; ++dim) {
+      for (int fdi = 0; fdi < 2; ++fdi) {
+        std::vector<int32_t> group;
+        for (int32_t i = 0; i < n; ++i)
+          if (items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1))) group.push_back(i);
+        if (group.empty()) continue;
+        LaunchCfg c;
+        int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &c);
+        if (st != VMC_OK) return st;
+        int64_t ne = 0, nb = 0;
+        std::vector<int64_t> tb(group.size()), bb(group.size());
+        for (size_t g = 0; g < group.size(); ++g) {
+          vmc_problem &m = *items[group[g]].prob;
+          upload_params(m, c.f64);
+          tb[g] = ne;
+          bb[g] = nb;
+          ne += sector_round(m.Ne);
+          nb += m.Nb;
+        }
+        if (ne > ((int64_t)1 << 30) / c.W || nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
+        arena_reset(g_batch_arena, c, ne, nb, group.size());
+        std::vector<ItemRec> recs;
+        for (size_t g = 0; g < group.size(); ++g) {
+          const vmc_batch_item &b = items[group[g]];
+          recs.push_back(make_item(*b.prob, &b.run, c.f64, (uint64_t)b.run.Nphoton, 0, tb[g], bb[g]));
+        }
+        int grid = 0;
+        kernel_ms += launch_items(g_batch_arena, m0->num_cus, c, recs, &grid);
+        std::vector<double> raw_e, raw_b;
+        std::vector<unsigned long long> loss;
+        double led[5];
+        unsigned long long cnt[4];
+        arena_collect(g_batch_arena, c, raw_e, raw_b, led, cnt, loss);
+        for (size_t g = 0; g < group.size(); ++g) {
+          vmc_batch_item &b = items[group[g]];
+          finish_outputs(*b.prob, b.run.f, b.run.Nphoton, c.W, raw_e.data() + tb[g] * c.W, raw_b.data() + bb[g] * 4,
+                         &b.out);
+          b.loss = (int64_t)loss[g];
+          tot.photons += b.run.Nphoton;
+        }
+        tot.steps += (int64_t)cnt[1];
+        tot.loss += (int64_t)cnt[2];
+        tot.rechecks += (int64_t)cnt[3];
+        for (int q = 0; q < 5; ++q) tot.ledger[q] += led[q];
+        tot.grid_blocks = grid;
+        tot.block_size = c.block;
+      }
+    }
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  if (stats) {
+    *stats = tot;
+    stats->kernel_ms = kernel_ms;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return VMC_OK;
 }
 
 void vmc_problem_destroy(vmc_problem *m) {
@@ -987,15 +1190,7 @@ void vmc_problem_destroy(vmc_problem *m) {
   m->cdf.release();
   m->dsrc.release();
   m->dbnd.release();
-  m->etally.release();
-  m->btally.release();
-  m->counters.release();
-  m->ledger.release();
-  m->raw_e.release();
-  m->raw_b.release();
-  if (m->ev0) (void)hipEventDestroy(m->ev0);
-  if (m->ev1) (void)hipEventDestroy(m->ev1);
-  if (m->stream) (void)hipStreamDestroy(m->stream);
+  m->arena.close();
   delete m;
 }
 

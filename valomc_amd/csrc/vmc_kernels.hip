@@ -64,8 +64,8 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 // sequences negligible.
 struct WalkRng {
   uint32_t s0, s1, s2, s3;
-  __device__ __forceinline__ void seed(uint32_t plo, uint32_t phi, const WalkParams &P) {
-    philox4x32_10(0x80000000u, P.tag, plo, phi, P.key0, P.key1, s0, s1, s2, s3);
+  __device__ __forceinline__ void seed(uint32_t plo, uint32_t phi, const ItemRec &it) {
+    philox4x32_10(0x80000000u, it.tag, plo, phi, it.key0, it.key1, s0, s1, s2, s3);
     if ((s0 | s1 | s2 | s3) == 0u) s0 = 1u;   // the all-zero state is a fixed point
   }
   __device__ __forceinline__ uint32_t next(const WalkParams &) {
@@ -348,15 +348,16 @@ template <class R> __device__ __forceinline__ R gauss_angle(R xi, R sig, R h) {
 // CreatePhoton (MC3D.hpp:1060-1294, MC2D.hpp:958-1132).  Returns the source's
 // mother element and initial face.
 template <int D, class R>
-__device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const WalkParams &P) {
+__device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const ItemRec &it, const WalkParams &P) {
   // source entry: first CDF value above xi, clamped (MC2D.hpp:966-969)
   const double xi = Unif<double>::closed(rs.next(P));
-  int lo = 0, hi = P.nsrc > 1 ? P.nsrc : 0;   // one source: nothing to search
+  const int nsrc = it.nsrc;
+  int lo = 0, hi = nsrc > 1 ? nsrc : 0;   // one source: nothing to search
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (xi < P.cdf[mid]) hi = mid; else lo = mid + 1;
+    if (xi < it.cdf[mid]) hi = mid; else lo = mid + 1;
   }
-  const SrcRec &s = P.src[lo < P.nsrc ? lo : P.nsrc - 1];
+  const SrcRec &s = it.src[lo < nsrc ? lo : nsrc - 1];
   p.el = s.mother;
   p.face = s.face;
   const int kind = s.kind;
@@ -442,8 +443,8 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const W
   p.w = R(1);
   p.acc0 = R(0);
   p.acc1 = R(0);
-  p.cph = R(P.cph0);
-  p.sph = R(P.sph0);
+  p.cph = R(it.cph0);
+  p.sph = R(it.sph0);
   p.nsteps = 0;
 }
 
@@ -477,14 +478,16 @@ template <> struct ExitWords<2, true> { static constexpr int n = 4; static const
 // each run costs one coalesced atomic request on one 64-byte line.  Device
 // atomics are served per line request (about 24 G requests/s on MI355X
 // whatever the word type), which makes requests, not words, the budget.
-template <int D, class R, bool FD, bool FIXED>
+// BATCH: items of different problems share the launch, so each lane carries
+// its photon's element records, tally base and wave-number factor; without
+// it every photon belongs to item 0 and these stay in scalar registers.
+template <int D, class R, bool FD, bool FIXED, bool BATCH>
 __global__ void __launch_bounds__(MaxBlock<R>::value)
     __attribute__((amdgpu_waves_per_eu(MinWaves<R>::value))) walk_kernel(const WalkParams P) {
   constexpr int NF = D + 1;
   constexpr int W = FD ? 2 : 1;
   typedef ExitWords<D, FD> XW;
   typedef typename Vec4T<R>::type V4;
-  const Elem<NF, R> *__restrict__ E = reinterpret_cast<const Elem<NF, R> *>(P.elems);
   const bool local = P.xcc_replicas != 0;
   const int rep = local ? (xcc_id() % P.nrep) : (int)(blockIdx.x % (unsigned)P.nrep);
   void *const etally = reinterpret_cast<char *>(P.etally) + (size_t)rep * P.ne * W * 8;
@@ -609,10 +612,27 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 #pragma unroll
   for (int j = 0; j < XW::n; ++j) pb[j] = R(0);
 
-  // wave-private photon id pool [pool_next, pool_end)
+  // the lane's photon: its item's element records, tally base and (FD)
+  // wave-number factor
+  const Elem<NF, R> *E = BATCH ? nullptr : reinterpret_cast<const Elem<NF, R> *>(P.items[0].elems);
+  int item = 0, tbase = BATCH ? 0 : P.items[0].tbase;
+  R kf = BATCH ? R(0) : R(P.items[0].kfac);
+
+  // wave-private pool of virtual photon ids [pool_next, pool_end), all of
+  // item pool_item; a chunk never straddles two items
   uint64_t pool_next = 0, pool_end = 0;
+  int pool_item = 0;
   bool exhausted = false;
   const int refill_min = P.refill_min;
+  // item owning virtual id v (items sorted by vstart; v < vtotal)
+  auto find_item = [&](uint64_t v) {
+    int lo = 0, hi = P.nitems;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (P.items[mid].vstart <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+  };
 
   for (;;) {
     if (!exhausted) {
@@ -621,31 +641,51 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       if (ndead >= refill_min) {
         const uint64_t avail = pool_end - pool_next;
         uint64_t fresh = 0;
+        int fresh_item = 0;
         if (avail < (uint64_t)ndead) {
           unsigned long long got = 0;
           if (lane == 0) got = atomicAdd(&P.counters[0], (unsigned long long)P.chunk);
           const uint32_t glo = __builtin_amdgcn_readfirstlane((uint32_t)got);
           const uint32_t ghi = __builtin_amdgcn_readfirstlane((uint32_t)(got >> 32));
           fresh = ((uint64_t)ghi << 32) | glo;
+          if (BATCH && fresh < P.vtotal) fresh_item = find_item(fresh);
         }
         const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(dead >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)dead, 0u));
         uint64_t pid = 0;
-        if (!alive) pid = (uint64_t)rank < avail ? pool_next + rank : fresh + (rank - avail);
+        int pit = pool_item;
+        if (!alive && (uint64_t)rank >= avail) {
+          pid = fresh + (rank - avail);
+          pit = fresh_item;
+        } else if (!alive) {
+          pid = pool_next + rank;
+        }
+        if constexpr (!BATCH) pit = 0;
         if (avail < (uint64_t)ndead) {
           pool_next = fresh + (ndead - avail);
           pool_end = fresh + (uint64_t)P.chunk;
+          pool_item = fresh_item;
         } else {
           pool_next += ndead;
         }
-        const bool invalid = !alive && pid >= P.n_end;
+        const bool invalid = !alive && pid >= P.vtotal;
         if (__ballot(invalid)) exhausted = true;
         if (!alive && !invalid) {
-          const uint32_t plo = (uint32_t)pid, phi = (uint32_t)(pid >> 32);
-          rs.seed(plo, phi, P);
-          create_photon<D, R>(p, rs, P);
-          p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
-          alive = true;
+          const ItemRec &it = P.items[pit];
+          const uint64_t k = pid - it.vstart;
+          if (k < it.count) {   // ids past the item's count pad its last chunk
+            const uint64_t sid = it.id0 + k;
+            rs.seed((uint32_t)sid, (uint32_t)(sid >> 32), it);
+            if constexpr (BATCH) {
+              item = pit;
+              E = reinterpret_cast<const Elem<NF, R> *>(it.elems);
+              tbase = it.tbase;
+              kf = R(it.kfac);
+            }
+            create_photon<D, R>(p, rs, it, P);
+            p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
+            alive = true;
+          }
         }
       }
     }
@@ -679,6 +719,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         end_visit();
         atomicAdd(&s_led[wid][3], (double)p.w);
         atomicAdd(&s_led[wid][4], 1.0);
+        atomicAdd(&P.item_loss[item], 1ull);
         alive = false;
       };
       // exit face: smallest positive plane distance over faces moving outward,
@@ -789,7 +830,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       } else {
         // w e^{-i phi} (1 - e^{-(mua + i k) ds}), MC3D.hpp:1519-1522, with the
         // phase advance b = k ds taken from half-angle terms
-        const R b = R(P.kfac) * nidx * ds;
+        const R b = kf * nidx * ds;
         R sh, ch;
         w_sincos_small(R(0.5) * b, &sh, &ch);
         const R sb = R(2) * sh * ch, ver = R(2) * sh * sh, cb = R(1) - ver;
@@ -835,7 +876,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         }
         // boundary (MC3D.hpp:1536-1567)
         const int ib = -1 - code;
-        const BndRec b = P.bnd[ib];
+        const BndRec b = P.items[item].bnd[ib];
         if (b.kind == kBndMirror) {
           R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
           if (D == 3) cd += p.d[2] * nrm[2];
@@ -850,7 +891,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         }
         // exitance (MC3D.hpp:1556-1564); 2D adds it weighted by
         // 1/(dir . outward normal) (MC2D.hpp:1410-1437)
-        pb_ib = ib;
+        pb_ib = P.items[item].bbase + ib;
         pb[0] = FD ? p.w * p.cph : p.w;
         if constexpr (FD) pb[1] = -p.w * p.sph;
         if constexpr (D == 2) {
@@ -908,10 +949,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
     // ---- commit the parked tally work (converged) ----
     {
-      const int es = pe_el >= 0 ? pe_el / kSectorElems : -1;
+      const int es = pe_el >= 0 ? (tbase + pe_el) / kSectorElems : -1;
       flush_sector(es >= 0 && sect >= 0 && es != sect);
       if (es >= 0) {
-        const int k = pe_el - es * kSectorElems;
+        const int k = tbase + pe_el - es * kSectorElems;
         sbuf[(k * W) * bstride] += pe0;
         if constexpr (FD) sbuf[(k * W + 1) * bstride] += pe1;
         sect = es;
@@ -998,27 +1039,29 @@ namespace vmc {
 
 typedef void (*WalkFn)(const WalkParams);
 
-static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed) {
-#define VMC_PICK(D, R)                                                                  \
-  return fd ? (fixed ? walk_kernel<D, R, true, true> : walk_kernel<D, R, true, false>)   \
-            : (fixed ? walk_kernel<D, R, false, true> : walk_kernel<D, R, false, false>)
-  if (dim == 3) {
-    if (f64) { VMC_PICK(3, double); } else { VMC_PICK(3, float); }
-  } else {
-    if (f64) { VMC_PICK(2, double); } else { VMC_PICK(2, float); }
-  }
-#undef VMC_PICK
+template <int D, class R>
+static WalkFn walk_fn_dr(bool fd, bool fixed, bool batch) {
+  if (batch)
+    return fd ? (fixed ? walk_kernel<D, R, true, true, true> : walk_kernel<D, R, true, false, true>)
+              : (fixed ? walk_kernel<D, R, false, true, true> : walk_kernel<D, R, false, false, true>);
+  return fd ? (fixed ? walk_kernel<D, R, true, true, false> : walk_kernel<D, R, true, false, false>)
+            : (fixed ? walk_kernel<D, R, false, true, false> : walk_kernel<D, R, false, false, false>);
 }
 
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, int block, size_t lds_bytes,
+static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed, bool batch) {
+  if (dim == 3) return f64 ? walk_fn_dr<3, double>(fd, fixed, batch) : walk_fn_dr<3, float>(fd, fixed, batch);
+  return f64 ? walk_fn_dr<2, double>(fd, fixed, batch) : walk_fn_dr<2, float>(fd, fixed, batch);
+}
+
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, int block, size_t lds_bytes,
                           int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed), block,
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed, batch), block,
                                                       lds_bytes);
 }
 
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, int grid, int block, size_t lds_bytes,
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, int grid, int block, size_t lds_bytes,
                        hipStream_t s, const WalkParams &P) {
-  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed), dim3(grid), dim3(block), lds_bytes, s, P);
+  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed, batch), dim3(grid), dim3(block), lds_bytes, s, P);
   return hipGetLastError();
 }
 

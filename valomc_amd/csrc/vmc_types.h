@@ -112,27 +112,44 @@ struct WalkLds {
 
 // Tally layout: nrep replicas of [Ne rounded up to a sector][cw ? 1 : 2]
 // element and [Nb][4] boundary slots (EBR, EBI, DEBR, DEBI).
+// One instance of a launch: a resident problem (its element records, light
+// sources and boundary), a photon range, a random-stream key and a place in
+// the tally arena.  A launch walks the photons of all its items.
+struct alignas(16) ItemRec {
+  uint64_t vstart;          // first virtual photon id (a multiple of the chunk)
+  uint64_t count;           // photons of this item in the launch
+  uint64_t id0;             // stream id of its first photon
+  const void *elems;        // element records
+  const double *cdf;        // [nsrc] light-source CDF
+  const SrcRec *src;        // [nsrc]
+  const BndRec *bnd;        // [Nb]
+  int32_t tbase;            // first element slot in the tally arena (whole sectors)
+  int32_t bbase;            // first boundary slot in the tally arena
+  int32_t nsrc;
+  uint32_t key0, key1, tag; // stream key (seed) and tag
+  double kfac;              // omega / c0: k = kfac * n
+  double cph0, sph0;        // cos/sin of the initial phase
+};
+
+// Tally arena: nrep replicas of [ne][W] element words and [nb][4] boundary
+// words (EBR, EBI, DEBR, DEBI).
 struct WalkParams {
-  const void *elems;
-  const double *cdf;
-  const SrcRec *src;
-  const BndRec *bnd;
+  const ItemRec *items;
+  int32_t nitems;
+  uint64_t vtotal;              // virtual photon ids [0, vtotal) of this launch
   void *etally;
   void *btally;
   double *ledger;               // [5] absorbed, exited, killed, gained, lost
   unsigned long long *counters; // [4] photon dispenser, steps, loss, rechecks
-  int64_t ne, nb;               // ne: element count rounded up to a whole sector
-  uint64_t n_end;               // photons [*, n_end) of this run
-  int32_t nsrc, nrep;
-  uint32_t key0, key1, tag;
+  unsigned long long *item_loss; // [nitems] photons lost per item (MC3D::loss)
+  int64_t ne, nb;               // arena element slots (whole sectors) and boundary slots
+  int32_t nrep;
   uint32_t step_cap;
   int32_t refill_min;
   int32_t chunk;
   int32_t xcc_replicas;         // 1: replica = XCD id, workgroup-scope (L2) atomics
   int32_t lds_slots;            // LDS sector-cache slots per workgroup (0 = none)
   int32_t queue_len;            // LDS run-queue sectors per wave (0 = direct atomics)
-  double phase0, cph0, sph0;    // initial phase and its cos/sin
-  double kfac;                  // omega / c0: k = kfac * n
   double weight0, chance;
   double fix_scale;             // fixed-point scale (2^32)
 };
