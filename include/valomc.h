@@ -131,6 +131,25 @@ int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
    light-source faces. */
 int vmc_problem_check(const vmc_problem_desc *desc, int64_t *HN_out, int64_t *nsources);
 
+/* Overrides for a variant of a resident problem; NULL keeps the base's. */
+typedef struct vmc_variant_desc {
+  const double *mua, *mus, *g, *n;   /* [Ne]                                      */
+  const char *BCType;                /* [Nb]: light sources and boundary kinds   */
+  const double *BCLNormal;           /* [Nb*dim], as given to vmc_problem_create  */
+  const char *BCLightDirectionType;  /* [Nb]                                     */
+  const double *BCIntensity;         /* [Nb]                                     */
+  const double *GaussianSigma;       /* [Nb], 2D                                 */
+} vmc_variant_desc;
+
+/* A problem that shares the base's mesh (and, without medium overrides, its
+   device element records) and differs in light sources and/or optical
+   parameters: the many source positions and parameter sets an inverse
+   solver evaluates on one mesh.  No counterpart in the reference, which
+   rebuilds a whole MC3D object per configuration; the checks are those of
+   ErrorChecks() and Init() that the overridden inputs can fail.  Destroy
+   variants and base in any order. */
+int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *d, vmc_problem **out);
+
 /* The neighbour table used (reference HN convention), [Ne*(dim+1)]. */
 int vmc_problem_get_hn(const vmc_problem *prob, int64_t *HN);
 

@@ -96,3 +96,47 @@ def test_batch_rejects_mixed_precision(vmc):
         items = [(prob, dict(nphoton=100, precision=vmc.FP32)), (prob, dict(nphoton=100, precision=vmc.FP64))]
         with pytest.raises(vmc.VmcError):
             vmc.batch_run(items)
+
+
+def _fresh_equal(vmc, p_mod, variant, **kw):
+    with vmc.Problem(p_mod) as fresh:
+        a = fresh.run(20000, seed=4, tally=vmc.TALLY_FIXED64, **kw)
+    b = variant.run(20000, seed=4, tally=vmc.TALLY_FIXED64, **kw)
+    for k in OUT:
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_variants_equal_fresh_problems(vmc, precision):
+    """A variant (shared mesh and records, other medium or sources) gives the
+    bits of a problem created from scratch with the same inputs."""
+    prec = vmc.FP32 if precision == "fp32" else vmc.FP64
+    p = _golden("p3_cosinic_hetero_n")
+    with vmc.Problem(p) as base:
+        # medium override
+        p2 = _golden("p3_cosinic_hetero_n")
+        p2.mua = np.asarray(p2.mua) * 1.1
+        p2.mus = np.asarray(p2.mus) * 0.9
+        v = base.variant(mua=p2.mua, mus=p2.mus)
+        _fresh_equal(vmc, p2, v, precision=prec)
+        v.close()
+        # light-source override: move the source to other boundary faces
+        p3 = _golden("p3_cosinic_hetero_n")
+        bct = np.asarray(p3.BCType).copy()
+        src = np.nonzero(bct != ord("a"))[0]
+        bct[src] = ord("a")
+        bct[np.arange(0, p3.Nb, 7)] = ord("l")
+        p3.BCType = bct
+        v = base.variant(BCType=bct)
+        _fresh_equal(vmc, p3, v, precision=prec)
+        # the base is untouched by its variants
+        _fresh_equal(vmc, _golden("p3_cosinic_hetero_n"), base, precision=prec)
+        v.close()
+
+
+def test_variant_without_light_source_is_refused(vmc):
+    p = _golden("p3_pencil_sphere")
+    with vmc.Problem(p) as base:
+        with pytest.raises(vmc.VmcError) as ei:
+            base.variant(BCType=np.full(p.Nb, ord("a"), dtype=np.uint8))
+        assert ei.value.status == 1   # 1 + NO_LIGHTSOURCE

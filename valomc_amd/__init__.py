@@ -105,6 +105,7 @@ def load_library():
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
+        lib.vmc_problem_create_variant.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
         lib.vmc_problem_destroy.restype = None
         lib.vmc_status_string.restype = C.c_char_p
         lib.vmc_last_error.restype = C.c_char_p
@@ -169,6 +170,12 @@ def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float 
     return rd
 
 
+class VariantDesc(C.Structure):
+    _fields_ = [("mua", C.c_void_p), ("mus", C.c_void_p), ("g", C.c_void_p), ("n", C.c_void_p),
+                ("BCType", C.c_void_p), ("BCLNormal", C.c_void_p), ("BCLightDirectionType", C.c_void_p),
+                ("BCIntensity", C.c_void_p), ("GaussianSigma", C.c_void_p)]
+
+
 class BatchItem(C.Structure):
     _fields_ = [("prob", C.c_void_p), ("run", RunDesc), ("out", Output), ("loss", C.c_int64),
                 ("status", C.c_int32), ("reserved", C.c_int32)]
@@ -223,6 +230,33 @@ class Problem:
 
     def __exit__(self, *exc):
         self.close()
+
+    def variant(self, mua=None, mus=None, g=None, n=None, BCType=None, BCLNormal=None, BCLDirType=None,
+                BCIntensity=None, GaussianSigma=None) -> "Problem":
+        """vmc_problem_create_variant: same mesh, other light sources and/or
+        optical parameters (None keeps this problem's)."""
+        keep = []
+
+        def ptr(a, dtype):
+            if a is None:
+                return None
+            if isinstance(a, (str, bytes)):
+                a = np.frombuffer(a.encode() if isinstance(a, str) else a, dtype=np.uint8)
+            arr = _colmajor(a, dtype)
+            keep.append(arr)
+            return arr.ctypes.data
+
+        d = VariantDesc(ptr(mua, np.float64), ptr(mus, np.float64), ptr(g, np.float64), ptr(n, np.float64),
+                        ptr(BCType, np.uint8), ptr(BCLNormal, np.float64), ptr(BCLDirType, np.uint8),
+                        ptr(BCIntensity, np.float64), ptr(GaussianSigma, np.float64))
+        h = C.c_void_p()
+        st = self.lib.vmc_problem_create_variant(self._h, C.byref(d), C.byref(h))
+        if st:
+            raise VmcError(st, self.lib)
+        v = Problem.__new__(Problem)
+        v.lib, v.dim, v.Ne, v.Nb, v.Nr = self.lib, self.dim, self.Ne, self.Nb, self.Nr
+        v._h = h
+        return v
 
     def _outputs(self):
         out = dict(ER=np.zeros(self.Ne), EI=np.zeros(self.Ne), EBR=np.zeros(self.Nb),

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -113,30 +114,59 @@ struct Arena {
 
 } // namespace
 
+// Mesh geometry and what is derived from it alone: shared by a problem and
+// its variants, immutable once the problem is created.
+struct MeshData {
+  std::vector<int64_t> H, HN, BH;
+  std::vector<double> r;
+  std::vector<double> vol, area;    // element volumes (2D areas), boundary areas (2D lengths)
+  std::vector<i4> nbcode;           // per element (caller's numbering), neighbours as device ids
+  std::vector<int32_t> perm, inv;   // device order: perm[device id] = element, inv[element] = device id
+};
+
+// Optical parameters and the device element records packed from them (with
+// the geometry): shared by variants that keep the medium, copied when one of
+// them changes it.
+struct MediumData {
+  std::vector<double> mua, mus, g, n;
+  DevBuf<unsigned char> elems32, elems64;
+  bool dirty32 = true, dirty64 = true;
+  ~MediumData() {
+    elems32.release();
+    elems64.release();
+  }
+};
+
 struct vmc_problem {
+  std::shared_ptr<MeshData> md;
+  std::shared_ptr<MediumData> med;
   int dim = 3, nv = 4, nbv = 3;
   int64_t Ne = 0, Nb = 0, Nr = 0;
-  std::vector<int64_t> H, HN, BH;
-  std::vector<double> r, mua, mus, g, n;
+  std::vector<int64_t> &H, &HN, &BH;
+  std::vector<double> &r;
   std::vector<char> BCType, BCLDirType;
   std::vector<double> BCn, BCLNormal, BCIntensity, GaussianSigma;
+  std::vector<double> BCLNormal_in;  // BCLNormal as given, before its preprocessing
   double shift[3] = {0, 0, 0};
-  std::vector<double> vol, area;    // element volumes (2D areas), boundary areas (2D lengths)
+  std::vector<double> &vol, &area;
   std::vector<int64_t> lsFace, lsMother;
   std::vector<double> lsCDF;
   std::vector<SrcRec> src;
   std::vector<BndRec> bnd;
-  std::vector<i4> nbcode;            // per element (caller's numbering), neighbours as device ids
-  std::vector<int32_t> perm, inv;    // device order: perm[device id] = element, inv[element] = device id
+  std::vector<i4> &nbcode;
+  std::vector<int32_t> &perm, &inv;
 
   int device = 0, num_cus = 0;
   int64_t lds_per_cu = 65536;
-  DevBuf<unsigned char> elems32, elems64;
   DevBuf<double> cdf;
   DevBuf<SrcRec> dsrc;
   DevBuf<BndRec> dbnd;
   Arena arena;   // tallies and stream of this problem's own runs
-  bool params_dirty32 = true, params_dirty64 = true;
+
+  explicit vmc_problem(std::shared_ptr<MeshData> mesh = std::make_shared<MeshData>(),
+                       std::shared_ptr<MediumData> medium = std::make_shared<MediumData>())
+      : md(std::move(mesh)), med(std::move(medium)), H(md->H), HN(md->HN), BH(md->BH), r(md->r),
+        vol(md->vol), area(md->area), nbcode(md->nbcode), perm(md->perm), inv(md->inv) {}
 
   int64_t h(int64_t el, int c) const { return H[el + Ne * c]; }
   int64_t hn(int64_t el, int c) const { return HN[el + Ne * c]; }
@@ -587,16 +617,16 @@ void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
       E.pl[f].z = (R)nrm[2];
       E.pl[f].w = (R)off;
     }
-    E.opt.x = (R)m.mua[e];
-    E.opt.y = (R)m.mus[e];
-    E.opt.z = (R)m.n[e];
-    E.opt.w = (R)m.g[e];
+    E.opt.x = (R)m.med->mua[e];
+    E.opt.y = (R)m.med->mus[e];
+    E.opt.z = (R)m.med->n[e];
+    E.opt.w = (R)m.med->g[e];
     E.nb = m.nbcode[e];
     for (int f = 0; f < NF; ++f) {
       const int code = m.nbcode[e][f];
       const int64_t nx = code >= 0 ? (int64_t)m.perm[code >> 3] : -1;
-      E.nbo[f].x = nx >= 0 ? (R)m.mus[nx] : R(0);
-      E.nbo[f].y = nx >= 0 ? (R)m.n[nx] : R(0);
+      E.nbo[f].x = nx >= 0 ? (R)m.med->mus[nx] : R(0);
+      E.nbo[f].y = nx >= 0 ? (R)m.med->n[nx] : R(0);
     }
   }
 }
@@ -634,34 +664,34 @@ void build_boundary(vmc_problem &m) {
 }
 
 void upload_params(vmc_problem &m, bool f64) {
-  if (f64 && !m.params_dirty64) return;
-  if (!f64 && !m.params_dirty32) return;
+  if (f64 && !m.med->dirty64) return;
+  if (!f64 && !m.med->dirty32) return;
   if (m.dim == 3) {
     if (f64) {
       std::vector<Elem<4, double>> v;
       pack_elements<4, double>(m, v);
-      m.elems64.ensure(v.size() * sizeof(v[0]));
-      hip_check(hipMemcpy(m.elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+      m.med->elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
     } else {
       std::vector<Elem<4, float>> v;
       pack_elements<4, float>(m, v);
-      m.elems32.ensure(v.size() * sizeof(v[0]));
-      hip_check(hipMemcpy(m.elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+      m.med->elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
     }
   } else {
     if (f64) {
       std::vector<Elem<3, double>> v;
       pack_elements<3, double>(m, v);
-      m.elems64.ensure(v.size() * sizeof(v[0]));
-      hip_check(hipMemcpy(m.elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+      m.med->elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
     } else {
       std::vector<Elem<3, float>> v;
       pack_elements<3, float>(m, v);
-      m.elems32.ensure(v.size() * sizeof(v[0]));
-      hip_check(hipMemcpy(m.elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+      m.med->elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
     }
   }
-  (f64 ? m.params_dirty64 : m.params_dirty32) = false;
+  (f64 ? m.med->dirty64 : m.med->dirty32) = false;
 }
 
 // Normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703)
@@ -670,7 +700,7 @@ void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t
   const bool d3 = m.dim == 3;
   if (omega <= 0.0) {
     for (int64_t i = 0; i < m.Ne; ++i) {
-      if (m.mua[i] > 0.0) ER[i] /= m.mua[i] * m.vol[i] * (double)N;
+      if (m.med->mua[i] > 0.0) ER[i] /= m.med->mua[i] * m.vol[i] * (double)N;
       else ER[i] /= m.vol[i] * (double)N;
     }
     for (int64_t i = 0; i < m.Nb; ++i) EBR[i] /= (double)N * m.area[i];
@@ -679,8 +709,8 @@ void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t
   } else {
     for (int64_t i = 0; i < m.Ne; ++i) {
       const double a = ER[i], b = EI[i], k = kfac_n[i];
-      ER[i] = (b * k + a * m.mua[i]) / (pow(k, 2) + pow(m.mua[i], 2)) / (double)N / m.vol[i];
-      EI[i] = -(a * k - b * m.mua[i]) / (pow(k, 2) + pow(m.mua[i], 2)) / (double)N / m.vol[i];
+      ER[i] = (b * k + a * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
+      EI[i] = -(a * k - b * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
     }
     for (int64_t i = 0; i < m.Nb; ++i) {
       EBR[i] /= (double)N * m.area[i];
@@ -722,10 +752,10 @@ int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
     copy_in(m->HN, d->HN, d->Ne * m->nv);
     copy_in(m->BH, d->BH, d->Nb * m->nbv);
     copy_in(m->r, d->r, d->Nr * m->dim);
-    copy_in(m->mua, d->mua, d->Ne);
-    copy_in(m->mus, d->mus, d->Ne);
-    copy_in(m->g, d->g, d->Ne);
-    copy_in(m->n, d->n, d->Ne);
+    copy_in(m->med->mua, d->mua, d->Ne);
+    copy_in(m->med->mus, d->mus, d->Ne);
+    copy_in(m->med->g, d->g, d->Ne);
+    copy_in(m->med->n, d->n, d->Ne);
     copy_in(m->BCType, d->BCType, d->Nb);
     copy_in(m->BCn, d->BCn, d->Nb);
     copy_in(m->BCLNormal, d->BCLNormal, d->Nb * m->dim);
@@ -738,6 +768,7 @@ int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
     if (m->BCIntensity.empty()) m->BCIntensity.assign(m->Nb, 1.0);
     if (m->BCLDirType.empty()) m->BCLDirType.assign(m->Nb, 'n');
     build_light_sources(*m);
+    m->BCLNormal_in = m->BCLNormal;
     prepare_directions(*m);
 
     for (int dd = 0; dd < m->dim; ++dd) {
@@ -925,7 +956,7 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
   ItemRec it{};
   it.count = count;
   it.id0 = id0;
-  it.elems = f64 ? (const void *)m.elems64.p : (const void *)m.elems32.p;
+  it.elems = f64 ? (const void *)m.med->elems64.p : (const void *)m.med->elems32.p;
   it.cdf = m.cdf.p;
   it.src = m.dsrc.p;
   it.bnd = m.dbnd.p;
@@ -955,7 +986,7 @@ void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const doub
     const int64_t dv = m.inv[i];
     ER[i] = raw_e[W * dv];
     if (fd) EI[i] = raw_e[W * dv + 1];
-    kn[i] = omega / kC0 * m.n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+    kn[i] = omega / kC0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
   }
   for (int64_t i = 0; i < m.Nb; ++i) {
     EBR[i] = raw_b[4 * i];
@@ -1030,14 +1061,89 @@ int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
   return VMC_OK;
 }
 
+int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *d, vmc_problem **out) {
+  if (!base || !d || !out) return VMC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::shared_ptr<MediumData> med = base->med;
+  if (d->mua || d->mus || d->g || d->n) {
+    med = std::make_shared<MediumData>();
+    med->mua = d->mua ? std::vector<double>(d->mua, d->mua + base->Ne) : base->med->mua;
+    med->mus = d->mus ? std::vector<double>(d->mus, d->mus + base->Ne) : base->med->mus;
+    med->g = d->g ? std::vector<double>(d->g, d->g + base->Ne) : base->med->g;
+    med->n = d->n ? std::vector<double>(d->n, d->n + base->Ne) : base->med->n;
+  }
+  vmc_problem *m = new vmc_problem(base->md, med);
+  m->dim = base->dim;
+  m->nv = base->nv;
+  m->nbv = base->nbv;
+  m->Ne = base->Ne;
+  m->Nb = base->Nb;
+  m->Nr = base->Nr;
+  for (int k = 0; k < 3; ++k) m->shift[k] = base->shift[k];
+  m->device = base->device;
+  m->num_cus = base->num_cus;
+  m->lds_per_cu = base->lds_per_cu;
+  const int64_t Nb = base->Nb;
+  m->BCn = base->BCn;
+  m->BCType = d->BCType ? std::vector<char>(d->BCType, d->BCType + Nb) : base->BCType;
+  m->BCLDirType = d->BCLightDirectionType ? std::vector<char>(d->BCLightDirectionType, d->BCLightDirectionType + Nb)
+                                          : base->BCLDirType;
+  m->BCIntensity = d->BCIntensity ? std::vector<double>(d->BCIntensity, d->BCIntensity + Nb) : base->BCIntensity;
+  m->GaussianSigma = d->GaussianSigma ? std::vector<double>(d->GaussianSigma, d->GaussianSigma + Nb)
+                                      : base->GaussianSigma;
+  m->BCLNormal = d->BCLNormal ? std::vector<double>(d->BCLNormal, d->BCLNormal + Nb * base->dim) : base->BCLNormal_in;
+  m->BCLNormal_in = m->BCLNormal;
+  try {
+    // the checks of ErrorChecks() that a source or medium change can fail
+    bool has_ls = false, has_gauss = false;
+    for (char c : m->BCType) {
+      if (c != 'a') has_ls = true;
+      if (c == 'g') has_gauss = true;
+    }
+    if (!has_ls) throw McError{NO_LIGHTSOURCE};
+    if (m->dim == 2 && has_gauss && m->GaussianSigma.empty()) throw McError{NO_GAUSSIAN_SIGMA};
+    build_light_sources(*m);
+    prepare_directions(*m);
+    build_source_records(*m);
+    build_boundary(*m);
+  } catch (McError e) {
+    delete m;
+    return e.code + 1;
+  }
+  try {
+    hip_check(hipSetDevice(m->device), "hipSetDevice");
+    m->arena.open();
+    m->cdf.ensure(m->lsCDF.size());
+    hip_check(hipMemcpy(m->cdf.p, m->lsCDF.data(), m->lsCDF.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    m->dsrc.ensure(m->src.size());
+    hip_check(hipMemcpy(m->dsrc.p, m->src.data(), m->src.size() * sizeof(SrcRec), hipMemcpyHostToDevice), "upload");
+    m->dbnd.ensure(m->bnd.size());
+    hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
+  } catch (HipFail f) {
+    int st = fail_hip(f);
+    vmc_problem_destroy(m);
+    return st;
+  }
+  *out = m;
+  return VMC_OK;
+}
+
 int vmc_problem_set_medium(vmc_problem *m, const double *mua, const double *mus, const double *g,
                            const double *n) {
   if (!m) return VMC_ERR_INVALID_ARGUMENT;
-  if (mua) m->mua.assign(mua, mua + m->Ne);
-  if (mus) m->mus.assign(mus, mus + m->Ne);
-  if (g) m->g.assign(g, g + m->Ne);
-  if (n) m->n.assign(n, n + m->Ne);
-  m->params_dirty32 = m->params_dirty64 = true;
+  if (m->med.use_count() > 1) {   // shared with a variant: copy on write
+    auto own = std::make_shared<MediumData>();
+    own->mua = m->med->mua;
+    own->mus = m->med->mus;
+    own->g = m->med->g;
+    own->n = m->med->n;
+    m->med = own;
+  }
+  if (mua) m->med->mua.assign(mua, mua + m->Ne);
+  if (mus) m->med->mus.assign(mus, mus + m->Ne);
+  if (g) m->med->g.assign(g, g + m->Ne);
+  if (n) m->med->n.assign(n, n + m->Ne);
+  m->med->dirty32 = m->med->dirty64 = true;
   return VMC_OK;
 }
 
@@ -1185,8 +1291,6 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
 
 void vmc_problem_destroy(vmc_problem *m) {
   if (!m) return;
-  m->elems32.release();
-  m->elems64.release();
   m->cdf.release();
   m->dsrc.release();
   m->dbnd.release();

@@ -221,6 +221,43 @@ def sphere_box_3d(nphoton: int = 100_000_000, seed: int = 1234, side_mm: int = 4
     return p
 
 
+def batched_box_instances(seed: int = 1234, photon_scale: float = 1.0):
+    """Batched forward solves on the 384k-tet box of ``sphere_box_3d``:
+    64 pencil positions on an 8x8 lattice of top-face points (5 mm cell
+    centres offset by (0.3, 0.2) mm off the vertices) x 4 optical-parameter
+    sets (mua and mus of every element times 1 + U(-0.1, 0.1), seeded) = 256
+    instances, photon counts log-uniform in [1e6, 1e8] (seeded) times
+    ``photon_scale``.
+
+    Returns (base problem, sources, media, photons): ``sources`` is a list of
+    64 dicts of BCType / BCLDirType / BCLNormal overrides, ``media`` a list
+    of 4 (mua, mus) pairs and ``photons`` an int array [4, 64]."""
+    base = sphere_box_3d(nphoton=1, seed=seed)
+    half = 20.0
+    rng = np.random.default_rng(seed + 1)
+    sources = []
+    for i in range(8):
+        for j in range(8):
+            x, y = -17.5 + 5.0 * i + 0.3, -17.5 + 5.0 * j + 0.2
+            r = np.asarray(base.r)
+            BH = np.asarray(base.BH)
+            ib = locate_top_triangle(r, BH, x, y, half)
+            bct = np.full(base.Nb, ord("a"), dtype=np.uint8)
+            bct[ib] = ord("p")
+            ldt = np.full(base.Nb, ord("n"), dtype=np.uint8)
+            ldt[ib] = ord("p")
+            bcl = np.zeros((base.Nb, 3))
+            bcl[ib] = (x, y, half)
+            sources.append(dict(BCType=bct, BCLDirType=ldt, BCLNormal=bcl, pencil=(x, y, half), face=int(ib)))
+    media = []
+    for _ in range(4):
+        mua = np.asarray(base.mua) * (1.0 + rng.uniform(-0.1, 0.1, base.Ne))
+        mus = np.asarray(base.mus) * (1.0 + rng.uniform(-0.1, 0.1, base.Ne))
+        media.append((mua, mus))
+    photons = np.round(10.0 ** rng.uniform(6.0, 8.0, (4, 64)) * photon_scale).astype(np.int64)
+    return base, sources, media, photons
+
+
 def locate_top_triangle(r, BH, x, y, ztop) -> int:
     """Boundary triangle on the plane z=ztop that contains (x, y)."""
     tri = r[BH]                                     # (Nb, 3, 3)
