@@ -196,6 +196,50 @@ int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr,
                     int32_t precision, double *ER, double *EI, double *EBR, double *EBI,
                     double *DEBR, double *DEBI, int64_t *HN_out, int64_t *loss);
 
+/* ------------------------------------------------------------------------
+   Serving: continuous batching of forward solves.  No counterpart in the
+   reference; for callers that issue many MonteCarlo() solves as they come
+   (an inverse-problem loop, a request queue).  A worker thread owns the GPU
+   work.  Requests are admitted in arrival order when their tallies fit in the
+   server's device arena; every launch advances all admitted requests of one
+   (dimension, CW/FD) group by a share of `launch_photons`, so requests that
+   arrive while others run join at the next launch.  Each request's photons
+   keep their stream ids (seed, stream_tag, photon index), so its result equals
+   vmc_problem_run's for the same inputs (bit for bit with
+   VMC_TALLY_FIXED64).  A problem and its output buffers belong to the server
+   from submit until wait returns for that ticket.
+   ------------------------------------------------------------------------ */
+typedef struct vmc_server vmc_server;
+
+typedef struct vmc_server_desc {
+  int32_t precision;      /* VMC_FP64 | VMC_FP32, for every request              */
+  int32_t tally;          /* VMC_TALLY_FP64 | VMC_TALLY_FIXED64                  */
+  int64_t arena_bytes;    /* device memory for admitted requests' tallies, all
+                             replicas; 0 = 4 GiB                                  */
+  int64_t launch_photons; /* photons per launch, shared by the requests in it;
+                             0 = 2e7 (about 0.15 s of FP32 work)                  */
+  int64_t min_slice;      /* least photons a request gets in a launch; 0 = 262144 */
+  int32_t reserved[8];
+} vmc_server_desc;
+
+int vmc_server_create(const vmc_server_desc *desc, vmc_server **out);
+/* Queue one solve; run->precision and run->tally are the server's.  out's
+   buffers are written before vmc_server_wait returns for the ticket.
+   VMC_ERR_OUT_OF_MEMORY if the request could never fit in the arena. */
+int vmc_server_submit(vmc_server *srv, vmc_problem *prob, const vmc_run_desc *run, const vmc_output *out,
+                      uint64_t *ticket);
+/* done = 1 once the ticket's outputs are written */
+int vmc_server_poll(vmc_server *srv, uint64_t ticket, int32_t *done);
+/* Blocks until the ticket is done and releases it; stats (optional) gets
+   photons, loss, kernel_ms of the launches it took part in and total_ms
+   from submit to completion.  Returns the request's status. */
+int vmc_server_wait(vmc_server *srv, uint64_t ticket, vmc_stats *stats);
+/* Launches so far, photons launched, summed kernel time, queue lengths. */
+int vmc_server_counters(vmc_server *srv, int64_t *launches, int64_t *photons, double *busy_ms,
+                        int32_t *waiting, int32_t *active);
+/* Finishes the queued and running requests, then frees the server. */
+void vmc_server_destroy(vmc_server *srv);
+
 const char *vmc_status_string(int status);
 /* last device error text of the calling thread */
 const char *vmc_last_error(void);

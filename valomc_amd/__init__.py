@@ -106,6 +106,13 @@ def load_library():
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
         lib.vmc_problem_create_variant.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.vmc_server_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+        lib.vmc_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
+        lib.vmc_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]
+        lib.vmc_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(Stats)]
+        lib.vmc_server_counters.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+        lib.vmc_server_destroy.argtypes = [C.c_void_p]
+        lib.vmc_server_destroy.restype = None
         lib.vmc_problem_destroy.restype = None
         lib.vmc_status_string.restype = C.c_char_p
         lib.vmc_last_error.restype = C.c_char_p
@@ -330,6 +337,79 @@ def batch_run(items, **common):
                              grid_blocks=int(s.grid_blocks), block_size=int(s.block_size),
                              rechecks=int(s.rechecks), **out))
     return res, s
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("tally", C.c_int32), ("arena_bytes", C.c_int64),
+                ("launch_photons", C.c_int64), ("min_slice", C.c_int64), ("reserved", C.c_int32 * 8)]
+
+
+class Server:
+    """vmc_server_*: continuous batching of forward solves on resident
+    problems.  ``submit`` returns a ticket; ``wait`` returns its RunResult
+    (``kernel_ms`` = launches it took part in, ``total_ms`` = submit to
+    completion)."""
+
+    def __init__(self, precision: int = FP32, tally: int = TALLY_FP64, arena_bytes: int = 0,
+                 launch_photons: int = 0, min_slice: int = 0):
+        self.lib = load_library()
+        d = ServerDesc(int(precision), int(tally), int(arena_bytes), int(launch_photons), int(min_slice))
+        h = C.c_void_p()
+        st = self.lib.vmc_server_create(C.byref(d), C.byref(h))
+        if st:
+            raise VmcError(st, self.lib)
+        self._h = h
+        self._pending = {}
+
+    def submit(self, prob: "Problem", nphoton: int, **kw) -> int:
+        rd = make_run_desc(nphoton, **kw)
+        out, o = prob._outputs()
+        t = C.c_uint64()
+        st = self.lib.vmc_server_submit(self._h, prob._h, C.byref(rd), C.byref(o), C.byref(t))
+        if st:
+            raise VmcError(st, self.lib)
+        self._pending[t.value] = (out, o, int(nphoton))
+        return t.value
+
+    def done(self, ticket: int) -> bool:
+        d = C.c_int32()
+        st = self.lib.vmc_server_poll(self._h, C.c_uint64(ticket), C.byref(d))
+        if st:
+            raise VmcError(st, self.lib)
+        return bool(d.value)
+
+    def wait(self, ticket: int) -> RunResult:
+        s = Stats()
+        st = self.lib.vmc_server_wait(self._h, C.c_uint64(ticket), C.byref(s))
+        out, _, n = self._pending.pop(ticket)
+        if st:
+            raise VmcError(st, self.lib)
+        return RunResult(status=st, loss=int(s.loss), steps=0, kernel_ms=float(s.kernel_ms),
+                         total_ms=float(s.total_ms), ledger=np.zeros(5), photons=n, grid_blocks=0,
+                         block_size=0, **out)
+
+    def counters(self):
+        l, p, b = C.c_int64(), C.c_int64(), C.c_double()
+        w, a = C.c_int32(), C.c_int32()
+        self.lib.vmc_server_counters(self._h, C.byref(l), C.byref(p), C.byref(b), C.byref(w), C.byref(a))
+        return dict(launches=l.value, photons=p.value, busy_ms=b.value, waiting=w.value, active=a.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.vmc_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class _EngineBase:

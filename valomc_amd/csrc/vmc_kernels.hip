@@ -1006,24 +1006,27 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 // Helper kernels
 // ---------------------------------------------------------------------------
 
-__global__ void fill_u64_kernel(unsigned long long *p, size_t n, unsigned long long v) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// p[r * stride + i] = v for r < nrep, i < n
+__global__ void fill_u64_kernel(unsigned long long *p, size_t n, int nrep, size_t stride, unsigned long long v) {
+  const size_t total = n * (size_t)nrep;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+    p[(i / n) * stride + i % n] = v;
 }
 
-// Sum the tally replicas in a fixed order: out[i] = sum_r rep[r][i].
+// Sum the tally replicas in a fixed order: out[i] = sum_r rep[r * stride + i].
 template <bool FIXED>
-__global__ void reduce_replicas_kernel(const void *rep, double *out, size_t n, int nrep, double inv_scale) {
+__global__ void reduce_replicas_kernel(const void *rep, double *out, size_t n, int nrep, size_t stride,
+                                       double inv_scale) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     if (FIXED) {
       const long long *r = reinterpret_cast<const long long *>(rep);
       long long s = 0;
-      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * n + i];
+      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * stride + i];
       out[i] = (double)s * inv_scale;
     } else {
       const double *r = reinterpret_cast<const double *>(rep);
       double s = 0.0;
-      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * n + i];
+      for (int k = 0; k < nrep; ++k) s += r[(size_t)k * stride + i];
       out[i] = s;
     }
   }
@@ -1065,25 +1068,32 @@ hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, int g
   return hipGetLastError();
 }
 
-hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
-  if (n == 0) return hipSuccess;
+hipError_t launch_fill_ranges(unsigned long long *p, size_t n, int nrep, size_t stride, unsigned long long v,
+                              hipStream_t s) {
+  if (n == 0 || nrep <= 0) return hipSuccess;
   const int block = 256;
-  const size_t want = (n + block - 1) / block;
+  const size_t want = (n * (size_t)nrep + block - 1) / block;
   const int grid = (int)(want < 4096 ? want : 4096);
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid), dim3(block), 0, s, p, n, v);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid), dim3(block), 0, s, p, n, nrep, stride, v);
   return hipGetLastError();
 }
 
-hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, double inv_scale,
-                         hipStream_t s) {
+hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s) {
+  return launch_fill_ranges(p, n, 1, n, v, s);
+}
+
+hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, size_t stride,
+                         double inv_scale, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int block = 256;
   const size_t want = (n + block - 1) / block;
   const int grid = (int)(want < 8192 ? want : 8192);
   if (fixed)
-    hipLaunchKernelGGL(reduce_replicas_kernel<true>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, inv_scale);
+    hipLaunchKernelGGL(reduce_replicas_kernel<true>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, stride,
+                       inv_scale);
   else
-    hipLaunchKernelGGL(reduce_replicas_kernel<false>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, inv_scale);
+    hipLaunchKernelGGL(reduce_replicas_kernel<false>, dim3(grid), dim3(block), 0, s, rep, out, n, nrep, stride,
+                       inv_scale);
   return hipGetLastError();
 }
 
