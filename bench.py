@@ -50,7 +50,10 @@ def parse():
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
-    ap.add_argument("--workload", choices=("box", "batch"), default="box",
+    ap.add_argument("--requests", type=int, default=300, help="serve workload: number of requests")
+    ap.add_argument("--utilization", type=float, default=0.8,
+                    help="serve workload: arrival rate as a fraction of the measured service rate")
+    ap.add_argument("--workload", choices=("box", "batch", "serve"), default="box",
                     help="box: the 384k-tet box (the headline line); batch: 256 instances "
                          "(64 pencil positions x 4 parameter sets) of the same box in one launch")
     ap.add_argument("--photon-scale", type=float, default=0.1,
@@ -179,6 +182,64 @@ def run_batch(a):
     print(json.dumps(line), flush=True)
 
 
+def run_serve(a):
+    """Poisson stream of forward solves through the serving loop: 40% 2D
+    meshes of 1k-200k triangles and 60% 3D boxes of 50k-2M tetrahedra from a
+    seeded pool of structured-then-jittered meshes kept resident, photons
+    lognormal (median 1e7, sigma 1), CW or 100 MHz half and half.  The rate
+    gives about `utilization` of the service rate measured per mesh first."""
+    import valomc_amd as vmc
+    from valomc_amd import meshgen, serving
+    pool = meshgen.serving_pool(seed=2024)
+    reqs = meshgen.serving_requests(a.requests, seed=2025)
+    cache = serving.MeshCache(budget_bytes=16 << 30)
+    kw = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
+    # service rate per (mesh, mode) from short single runs
+    rate = {}
+    for k, p in enumerate(pool):
+        prob = cache.get(p)
+        for f in (0.0, 100e6):
+            prob.run(200000, seed=1, f=f, **kw)
+            r = prob.run(2000000, seed=2, f=f, **kw)
+            rate[(p.notes["kind"], k % 4, f)] = 2000000 / (r.total_ms / 1e3)
+        cache.release(prob)
+    service = [q["photons"] / rate[(q["kind"], q["index"], q["f"])] for q in reqs]
+    lam = a.utilization / float(np.mean(service))
+    keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
+    with vmc.Server(arena_bytes=16 << 30, **kw) as srv:
+        out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
+        ctr = srv.counters()
+    lat = np.array([o["latency_ms"] for o in out])
+    total = int(sum(o["photons"] for o in out))
+    # each kept request's result alone against its result in the mix
+    by_kind = {"2d": [p for p in pool if p.notes["kind"] == "2d"], "3d": [p for p in pool if p.notes["kind"] == "3d"]}
+    diffs = []
+    for o in out:
+        if "ER" not in o:
+            continue
+        prob = cache.get(by_kind[o["kind"]][o["index"]])
+        alone = prob.run(o["photons"], seed=o["seed"], f=o["f"], **kw)
+        cache.release(prob)
+        diffs.append(float(np.linalg.norm(alone.ER - o["ER"]) / np.linalg.norm(alone.ER)))
+    cache.close()
+    line = {
+        "metric": METRIC, "value": round(total / wall, 1), "unit": "photons/s", "n_gpus": 1,
+        "steps": len(out), "warmup": 0, "ms_per_step": round(wall / len(out) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving: Poisson stream, 40% 2D (1k-200k triangles) / 60% 3D (50k-2M tets) "
+                               "resident jittered meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50",
+                   "requests": len(out), "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
+                   "photons_total": total, "parallelism": "single GPU"},
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+        "latency_ms_max": round(float(lat.max()), 2),
+        "wall_s": round(wall, 3), "gpu_busy_fraction": round(ctr["busy_ms"] / 1e3 / wall, 3),
+        "launches": ctr["launches"],
+        "alone_vs_mix_rel_l2_max": max(diffs) if diffs else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -190,6 +251,9 @@ def main():
         return
     if a.workload == "batch":
         run_batch(a)
+        return
+    if a.workload == "serve":
+        run_serve(a)
         return
     import valomc_amd as vmc
     from valomc_amd import meshgen

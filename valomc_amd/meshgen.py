@@ -343,3 +343,76 @@ def fd_box_3d(nphoton: int = 300_000_000, seed: int = 7, h: float = 0.75,
     p.notes.update(cells=(nx, ny, nz), corr_mm=corr_mm, sigma_mm=sigma_mm,
                    source_faces=int(src.sum()))
     return p
+
+
+def serving_pool(seed: int = 2024, scale: float = 1.0):
+    """Seeded pool of structured-then-jittered meshes for the serving mix:
+    four 2D squares of about 1k-200k triangles and four 3D boxes of about
+    50k-2M tetrahedra (interior nodes moved by U(+-0.2h)), each with a
+    lognormal smooth-field medium and a boundary source (2D: cosinic edges
+    on the left side; 3D: Gaussian-profile direct light on the top face,
+    sigma = 1/10 of the side).  ``scale`` < 1 shrinks every mesh (cell counts
+    times scale^(1/dim)) for quick runs.  Returns a list of Problems with
+    ``notes['kind']`` = '2d' or '3d'."""
+    pool = []
+    for k, n in enumerate((23, 50, 110, 316)):
+        n = max(4, int(round(n * scale ** 0.5)))
+        side = 10.0
+        h = side / n
+        v = (np.arange(n) + 0.5) * h - side / 2
+        r, H, BH = grid_mesh(v, v, h=h)
+        r = jitter_interior(r, h, 0.2, seed + k)
+        cen = r[H].mean(axis=1)
+        mua = lognormal_from_field(smooth_field(cen, 1.0, seed + 10 + k, grid_h=min(0.5, h)), 0.02, 0.3)
+        mus = lognormal_from_field(smooth_field(cen, 1.0, seed + 20 + k, grid_h=min(0.5, h)), 1.0, 0.2)
+        Ne, Nb = H.shape[0], BH.shape[0]
+        p = Problem(dim=2, H=H, BH=BH, r=r, mua=mua, mus=mus, g=np.full(Ne, 0.9), n=np.full(Ne, 1.33),
+                    BCType=_chars("a", Nb), BCn=np.full(Nb, 1.0))
+        seg = r[BH]
+        left = np.all(np.abs(seg[:, :, 0] - r[:, 0].min()) < 1e-9, axis=1) & (np.abs(seg[:, :, 1].mean(axis=1)) < 2.0)
+        p.BCType[left] = ord("c")
+        p.notes.update(kind="2d", cells=n)
+        pool.append(p)
+    for k, n in enumerate((21, 33, 52, 69)):
+        n = max(3, int(round(n * scale ** (1.0 / 3.0))))
+        side = 30.0
+        h = side / n
+        v = (np.arange(n) + 0.5) * h - side / 2
+        r, H, BH = grid_mesh(v, v, v, h=h)
+        r = jitter_interior(r, h, 0.2, seed + 100 + k)
+        cen = r[H].mean(axis=1)
+        gh = max(h, 0.5)
+        mua = lognormal_from_field(smooth_field(cen, 3.0, seed + 110 + k, grid_h=gh), 0.02, 0.3)
+        mus = lognormal_from_field(smooth_field(cen, 3.0, seed + 120 + k, grid_h=gh), 1.5, 0.2)
+        Ne, Nb = H.shape[0], BH.shape[0]
+        p = Problem(dim=3, H=H, BH=BH, r=r, mua=mua, mus=mus, g=np.full(Ne, 0.9), n=np.full(Ne, 1.37),
+                    BCType=_chars("a", Nb), BCn=np.full(Nb, 1.0))
+        ztop = r[:, 2].max()
+        tri = r[BH]
+        top = np.all(np.abs(tri[:, :, 2] - ztop) < 1e-9, axis=1)
+        rho = np.linalg.norm(tri[:, :, :2].mean(axis=1), axis=1)
+        sig = side / 10.0
+        src = top & (rho <= 4 * sig)
+        p.BCType[src] = ord("l")
+        p.BCIntensity = np.where(src, np.exp(-rho ** 2 / (2 * sig ** 2)), 0.0)
+        p.notes.update(kind="3d", cells=n)
+        pool.append(p)
+    return pool
+
+
+def serving_requests(n: int, seed: int = 2025, median: float = 1e7, sigma: float = 1.0,
+                     p2d: float = 0.4, p_fd: float = 0.5, freq_hz: float = 100e6):
+    """Seeded request mix for the serving loop: mesh kind 2D with probability
+    p2d (else 3D), a mesh of that kind drawn uniformly from the pool,
+    photons lognormal (median, sigma) clipped to [1e4, 1e9], CW or ``freq_hz``
+    with probability p_fd.  Returns a list of dicts (kind, index, photons, f,
+    seed); arrival times are drawn separately for a chosen rate."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        kind = "2d" if rng.random() < p2d else "3d"
+        idx = int(rng.integers(0, 4))
+        ph = int(np.clip(np.exp(rng.normal(math.log(median), sigma)), 1e4, 1e9))
+        f = freq_hz if rng.random() < p_fd else 0.0
+        out.append(dict(kind=kind, index=idx, photons=ph, f=f, seed=1 + i))
+    return out
