@@ -1479,38 +1479,81 @@ struct vmc_server {
   int turn = 0;
   int64_t launches = 0, photons = 0;
   double busy_ms = 0;
+  uint64_t epoch = 0;   // bumped by every submit and every release of arena ranges
   std::thread worker;
+  // finisher: reduces, copies and normalises completed requests on its own
+  // stream while the worker keeps launching, then returns their ranges
+  std::deque<Request *> to_finish;
+  std::condition_variable fin_cv;
+  bool fin_stop = false;
+  hipStream_t fstream = nullptr;
+  DevBuf<double> fraw_e, fraw_b;
+  std::thread finisher;
 };
 
 namespace {
 
 const int64_t kWordUnit = 16;   // arena element words are handed out in whole FD sectors
 
-void server_finish(vmc_server *s, const LaunchCfg &c, Request *r) {
+void server_finish(vmc_server *s, Request *r) {
   std::vector<double> raw_e((size_t)r->ewords), raw_b((size_t)r->prob->Nb * 4);
-  const double inv = c.fixed ? 1.0 / 4294967296.0 : 1.0;
+  const bool fixed = s->desc.tally == VMC_TALLY_FIXED64;
+  const double inv = fixed ? 1.0 / 4294967296.0 : 1.0;
   Arena &a = s->arena;
-  a.raw_e.ensure((size_t)r->ewords);
-  a.raw_b.ensure((size_t)r->prob->Nb * 4);
-  hip_check(launch_reduce(c.fixed, a.etally.p + r->eoff, a.raw_e.p, (size_t)r->ewords, s->nrep, (size_t)s->cap_words,
-                          inv, a.stream), "reduce");
-  hip_check(launch_reduce(c.fixed, a.btally.p + r->boff * 4, a.raw_b.p, (size_t)r->prob->Nb * 4, s->nrep,
-                          (size_t)s->cap_b * 4, inv, a.stream), "reduce");
-  hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, raw_e.size() * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
-  hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, raw_b.size() * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
-  hip_check(hipStreamSynchronize(a.stream), "sync");
+  s->fraw_e.ensure((size_t)r->ewords);
+  s->fraw_b.ensure((size_t)r->prob->Nb * 4);
+  hip_check(launch_reduce(fixed, a.etally.p + r->eoff, s->fraw_e.p, (size_t)r->ewords, s->nrep, (size_t)s->cap_words,
+                          inv, s->fstream), "reduce");
+  hip_check(launch_reduce(fixed, a.btally.p + r->boff * 4, s->fraw_b.p, (size_t)r->prob->Nb * 4, s->nrep,
+                          (size_t)s->cap_b * 4, inv, s->fstream), "reduce");
+  hip_check(hipMemcpyAsync(raw_e.data(), s->fraw_e.p, raw_e.size() * 8, hipMemcpyDeviceToHost, s->fstream), "d2h");
+  hip_check(hipMemcpyAsync(raw_b.data(), s->fraw_b.p, raw_b.size() * 8, hipMemcpyDeviceToHost, s->fstream), "d2h");
+  hip_check(hipStreamSynchronize(s->fstream), "sync");
   finish_outputs(*r->prob, r->run.f, r->run.Nphoton, r->fd ? 2 : 1, raw_e.data(), raw_b.data(), &r->out);
+}
+
+void finisher_loop(vmc_server *s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->fin_cv.wait(lk, [&] { return s->fin_stop || !s->to_finish.empty(); });
+      if (s->to_finish.empty()) return;
+      r = s->to_finish.front();
+      s->to_finish.pop_front();
+    }
+    if (r->status == VMC_OK) {
+      try {
+        server_finish(s, r);
+      } catch (HipFail f) {
+        r->status = fail_hip(f);
+      }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t ew = (r->ewords + kWordUnit - 1) / kWordUnit * kWordUnit;
+    s->ealloc.release(r->eoff, ew);
+    s->balloc.release(r->boff, r->prob->Nb);
+    s->epoch++;
+    r->finished = true;
+    r->t_done = std::chrono::steady_clock::now();
+    s->done_cv.notify_all();
+    s->cv.notify_all();
+  }
 }
 
 void server_loop(vmc_server *s) {
   (void)hipSetDevice(s->device);
+  uint64_t seen = ~0ull;   // epoch of the last admission attempt that found no room
   for (;;) {
     std::vector<Request *> admitted, group;
     int gdim = 3;
     bool gfd = false;
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&] { return s->stop || !s->waiting.empty() || !s->active.empty(); });
+      s->cv.wait(lk, [&] {
+        return (s->stop && s->waiting.empty()) || !s->active.empty() || (!s->waiting.empty() && s->epoch != seen);
+      });
       if (s->stop && s->waiting.empty() && s->active.empty()) return;
       // admission in arrival order while the tallies fit
       while (!s->waiting.empty()) {
@@ -1529,7 +1572,10 @@ void server_loop(vmc_server *s) {
         s->active.push_back(r);
         admitted.push_back(r);
       }
-      if (s->active.empty()) continue;
+      if (s->active.empty()) {
+        seen = s->epoch;   // nothing fits until the finisher returns ranges
+        continue;
+      }
       // the next group in turn among the (dimension, CW/FD) groups with work
       for (int k = 0; k < 4; ++k) {
         const int g = (s->turn + k) % 4;
@@ -1602,11 +1648,6 @@ void server_loop(vmc_server *s) {
           if (r->run.Nphoton == 0 && std::find(done_now.begin(), done_now.end(), r) == done_now.end())
             done_now.push_back(r);
       }
-      for (Request *r : done_now) {
-        LaunchCfg cr;
-        (void)resolve_cfg(r->prob->dim, s->num_cus, s->lds_per_cu, r->fd, &tmpl, &cr);
-        server_finish(s, cr, r);
-      }
     } catch (HipFail f) {
       const int st = fail_hip(f);
       std::lock_guard<std::mutex> lk(s->mu);
@@ -1615,16 +1656,14 @@ void server_loop(vmc_server *s) {
         if (std::find(done_now.begin(), done_now.end(), r) == done_now.end()) done_now.push_back(r);
       }
     }
+    // hand completed requests to the finisher; their last launch has
+    // completed, so their tallies are final
     std::lock_guard<std::mutex> lk(s->mu);
     for (Request *r : done_now) {
-      const int64_t ew = (r->ewords + kWordUnit - 1) / kWordUnit * kWordUnit;
-      s->ealloc.release(r->eoff, ew);
-      s->balloc.release(r->boff, r->prob->Nb);
       s->active.erase(std::find(s->active.begin(), s->active.end(), r));
-      r->finished = true;
-      r->t_done = std::chrono::steady_clock::now();
+      s->to_finish.push_back(r);
     }
-    if (!done_now.empty()) s->done_cv.notify_all();
+    if (!done_now.empty()) s->fin_cv.notify_all();
   }
 }
 
@@ -1662,6 +1701,7 @@ int vmc_server_create(const vmc_server_desc *d, vmc_server **out) {
     s->cap_b = std::min<int64_t>(per_rep / 8 / 4, (int64_t)1 << 28);
     if (s->cap_words <= 0 || s->cap_b <= 0) return VMC_ERR_INVALID_ARGUMENT;
     s->arena.open();
+    hip_check(hipStreamCreateWithFlags(&s->fstream, hipStreamNonBlocking), "hipStreamCreate");
     s->arena.W = 1;
     s->arena.nrep = s->nrep;
     s->arena.etally.ensure((size_t)s->cap_words * s->nrep);
@@ -1675,12 +1715,14 @@ int vmc_server_create(const vmc_server_desc *d, vmc_server **out) {
   } catch (HipFail f) {
     const int st = fail_hip(f);
     s->arena.close();
+    if (s->fstream) (void)hipStreamDestroy(s->fstream);
     return st;
   }
   s->ealloc.reset(s->cap_words);
   s->balloc.reset(s->cap_b);
   vmc_server *raw = s.release();
   raw->worker = std::thread(server_loop, raw);
+  raw->finisher = std::thread(finisher_loop, raw);
   *out = raw;
   return VMC_OK;
 }
@@ -1704,6 +1746,7 @@ int vmc_server_submit(vmc_server *s, vmc_problem *prob, const vmc_run_desc *run,
   std::lock_guard<std::mutex> lk(s->mu);
   r->ticket = s->next_ticket++;
   *ticket = r->ticket;
+  s->epoch++;
   s->waiting.push_back(r.get());
   s->all[r->ticket] = std::move(r);
   s->cv.notify_all();
@@ -1758,6 +1801,15 @@ void vmc_server_destroy(vmc_server *s) {
     s->cv.notify_all();
   }
   if (s->worker.joinable()) s->worker.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fin_stop = true;
+    s->fin_cv.notify_all();
+  }
+  if (s->finisher.joinable()) s->finisher.join();
+  s->fraw_e.release();
+  s->fraw_b.release();
+  if (s->fstream) (void)hipStreamDestroy(s->fstream);
   s->arena.close();
   delete s;
 }
