@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per element step: face planes + neighbour codes +
 # optical parameters of the element, plus the tally read-modify-write
 B_STEP = {("fp32", False): 112, ("fp32", True): 128, ("fp64", False): 192, ("fp64", True): 208}
+B_STEP_2D = 96   # 2D FP32 CW: 3 edge lines + neighbour codes + optics + tally
 
 
 def parse():
@@ -40,7 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons", type=float, default=1e8)
+    ap.add_argument("--photons", type=float, default=None,
+                    help="photons per step (default: 1e8 box, 3e8 fd, 1e6 square2d)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
@@ -53,7 +55,7 @@ def parse():
     ap.add_argument("--requests", type=int, default=300, help="serve workload: number of requests")
     ap.add_argument("--utilization", type=float, default=0.8,
                     help="serve workload: arrival rate as a fraction of the measured service rate")
-    ap.add_argument("--workload", choices=("box", "batch", "serve"), default="box",
+    ap.add_argument("--workload", choices=("box", "fd", "square2d", "batch", "serve"), default="box",
                     help="box: the 384k-tet box (the headline line); batch: 256 instances "
                          "(64 pencil positions x 4 parameter sets) of the same box in one launch")
     ap.add_argument("--photon-scale", type=float, default=0.1,
@@ -61,7 +63,11 @@ def parse():
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="only run a streaming copy kernel (out = -in, float32) over this many bytes "
                          "three times: the reference dispatch that scales FETCH_SIZE / WRITE_SIZE")
-    return ap.parse_args()
+    a = ap.parse_args()
+    a.photons_given = a.photons is not None
+    if a.photons is None:
+        a.photons = 1e8
+    return a
 
 
 def copy_reference(nbytes: int):
@@ -102,7 +108,7 @@ def cpu_model() -> str:
     return "unknown CPU"
 
 
-def cpu_baseline(p, target_s: float):
+def cpu_baseline(p, target_s: float, what: str = "the same problem"):
     """Time the reference engine (or, without it, the oracle) on the host."""
     # the OpenMP runtime reads these when the reference library first loads
     os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -120,7 +126,7 @@ def cpu_baseline(p, target_s: float):
     p.Nphoton = n
     r = run(p, threads)
     return {"value": n / r.seconds, "unit": "photons/s", "cores": threads, "kind": kind,
-            "sample": f"{n} photons of the same 384k-tet problem, MonteCarlo() only, "
+            "sample": f"{n} photons of {what}, MonteCarlo() only, "
                       f"{threads} OpenMP threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
                       f"OMP_PLACES={os.environ.get('OMP_PLACES')}) on {cpu_model()}, {r.seconds:.1f} s"}
 
@@ -258,11 +264,26 @@ def main():
     import valomc_amd as vmc
     from valomc_amd import meshgen
 
-    nphot = int(a.photons)
-    p = meshgen.sphere_box_3d(nphoton=nphot, seed=1234)
+    # single-problem workloads: the headline box, the FD box, the 2D square
+    if a.workload == "fd":
+        nphot = int(a.photons) if a.photons_given else 300_000_000
+        make = lambda n: meshgen.fd_box_3d(nphoton=n, seed=7)
+        desc = ("2,035,200-tet jittered 60x60x40 mm box, smooth-field medium, Gaussian-profile source, "
+                "100 MHz, FP32 transport + FP64 tallies")
+    elif a.workload == "square2d":
+        nphot = int(a.photons) if a.photons_given else 1_000_000
+        make = lambda n: meshgen.square_2d(nphoton=n, seed=1)
+        desc = "200-triangle 10x10 mm square, cosinic source on edges 4-7, CW, FP32 transport + FP64 tallies"
+    else:
+        nphot = int(a.photons)
+        make = lambda n: meshgen.sphere_box_3d(nphoton=n, seed=1234)
+        desc = "384k-tet heterogeneous box, pencil source, CW, FP32 transport + FP64 tallies"
+    p = make(nphot)
+    fd = float(getattr(p, "f", 0.0) or 0.0) > 0.0
     prob = vmc.Problem(p)
     kw = dict(block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, refill_min=a.refill_min,
-              replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
+              replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64,
+              f=float(getattr(p, "f", 0.0) or 0.0))
 
     for i in range(a.warmup):
         prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)
@@ -275,13 +296,14 @@ def main():
     value = a.steps * nphot / wall
     kernel_s = sum(r.kernel_ms for r in runs) / 1e3
     steps_total = sum(r.steps for r in runs)
-    achieved = steps_total * B_STEP[("fp32", False)] / kernel_s / 1e9
-    tps = traffic_per_step()
+    bstep = B_STEP[("fp32", fd)] if p.dim == 3 else B_STEP_2D
+    achieved = steps_total * bstep / kernel_s / 1e9
+    tps = traffic_per_step() if a.workload == "box" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if tps is None else round(tps * steps_total / a.steps),
-                "kernel": "walk_kernel<3,float,CW,fp64-tally>",
-                "bytes_per_step": B_STEP[("fp32", False)],
+                "kernel": f"walk_kernel<{p.dim},float,{'FD' if fd else 'CW'},fp64-tally>",
+                "bytes_per_step": bstep,
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
 
@@ -296,27 +318,29 @@ def main():
                  "fp32_speedup_vs_fp64": round((nphot / (wall / a.steps)) / (nphot / t64), 3),
                  "fp32_vs_fp64_rel_l2": rel,
                  "fp64_kernel_ms": round(f64.kernel_ms, 3)}
+        if fd:
+            extra["fp32_vs_fp64_rel_l2_EI"] = float(np.linalg.norm(f32.EI - f64.EI) / np.linalg.norm(f64.EI))
     prob.close()
 
     led = runs[-1].ledger
     balance = abs(led[0] + led[1] + led[2] - led[3] + led[4] - nphot) / nphot
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(meshgen.sphere_box_3d(nphoton=1, seed=1234), a.cpu_seconds)
+        cpu = cpu_baseline(make(1), a.cpu_seconds, desc.split(",")[0])
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "384k-tet heterogeneous box, pencil source, CW, FP32 transport + FP64 tallies",
-                   "elements": p.Ne, "boundary_faces": p.Nb, "nodes": p.Nr,
+        "config": {"workload": desc, "elements": p.Ne, "boundary_faces": p.Nb, "nodes": p.Nr,
                    "photons_per_step": nphot, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "element_steps_per_s": round(steps_total / kernel_s, 1),
         "energy_balance_rel": balance,
         "loss_per_step": runs[-1].loss,
+        "rechecks_per_step": runs[-1].rechecks,
         "grid": [runs[-1].grid_blocks, runs[-1].block_size],
         **extra,
     }
