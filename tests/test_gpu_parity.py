@@ -223,3 +223,43 @@ def test_one_shot_mex_style_entry(vmc):
     ref = pyoracle.run(p, 2, pyoracle.RNG_PHILOX)
     assert rel_l2(out["ER"], ref.ER) < 1e-9
     assert np.array_equal(hn.reshape(4, p.Ne).T, ref.HN)
+
+
+@pytest.mark.parametrize("name,nphot", [("p3_pencil_sphere", 100000), ("p2_square_cosinic", 200000),
+                                        ("p3_cosinic_hetero_n", 100000)])
+def test_per_element_z_scores_against_reference_streams(vmc, name, nphot):
+    """Element by element against the reference's own MT19937 streams: K
+    independent seeds on each side give a mean and a standard error per
+    element; over the elements whose mean is known to 5 % on both sides,
+    z = (gpu - ref) / sqrt(se_gpu^2 + se_ref^2) must look standard normal
+    (mean z^2 near 1, no |z| beyond 4.5).  A bias of a few percent anywhere
+    (a dropped term, a wrong branch) shows up as large z on the well-sampled
+    elements even when the global relative L2 test passes."""
+    K = 10
+    p = _golden(name)
+    g, r = [], []
+    with vmc.Problem(p) as prob:
+        for k in range(K):
+            g.append(prob.run(nphot, seed=1000 + k, f=p.f, phase0=p.phase0, precision=vmc.FP64).ER)
+    for k in range(K):
+        q = _golden(name)
+        # thread t of an MT run uses seed + t: space the seeds so no two
+        # runs share a stream
+        q.Nphoton, q.seed = nphot, 5000 + 16 * k
+        r.append(pyoracle.run(q, 8, pyoracle.RNG_MT19937).ER)
+    g, r = np.array(g), np.array(r)
+    mg, mr = g.mean(0), r.mean(0)
+    sg, sr = g.std(0, ddof=1) / np.sqrt(K), r.std(0, ddof=1) / np.sqrt(K)
+    ok = (mg > 0) & (mr > 0) & (sg < 0.05 * mg) & (sr < 0.05 * mr)
+    assert ok.sum() >= 50, ok.sum()
+    z = (mg[ok] - mr[ok]) / np.sqrt(sg[ok] ** 2 + sr[ok] ** 2)
+    # the standard errors are estimated from K runs a side, so z follows a
+    # t distribution with about 2(K-1) degrees of freedom; the largest |z|
+    # over n elements is bounded by the Bonferroni quantile at 1e-3
+    from scipy import stats
+    nu = 2 * (K - 1)
+    zmax = stats.t.ppf(1.0 - 1e-3 / (2 * ok.sum()), nu)
+    print(f"{name}: {ok.sum()} elements, mean z^2 {np.mean(z ** 2):.3f} (t: {nu / (nu - 2):.3f}), "
+          f"max |z| {np.abs(z).max():.2f} (bound {zmax:.2f})", flush=True)
+    assert 0.5 < np.mean(z ** 2) < 2.0, np.mean(z ** 2)
+    assert np.abs(z).max() < zmax, np.abs(z).max()
