@@ -214,13 +214,22 @@ template <class R> struct Photon {
   int el, face, nsteps;
 };
 
+// Element records, light sources and boundary records live in device
+// global memory.  Their pointers are read from an item record, so without
+// this address space the compiler would access them with flat instructions,
+// which also wait on the LDS counter.
+template <int NF, class R> using ElemPtr = const __attribute__((address_space(1))) Elem<NF, R> *;
+template <class T> __device__ __forceinline__ const __attribute__((address_space(1))) T *global_ptr(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+
 // Load the record of element `el` into registers.
 template <int NF, class R>
-__device__ __forceinline__ void load_elem(const Elem<NF, R> *__restrict__ E, int el,
+__device__ __forceinline__ void load_elem(ElemPtr<NF, R> E, int el,
                                           typename Vec4T<R>::type (&pl)[NF],
                                           typename Vec4T<R>::type &opt, i4 &nb,
                                           typename Vec2T<R>::type (&nbo)[NF]) {
-  const Elem<NF, R> *e = E + el;
+  ElemPtr<NF, R> e = E + el;
 #pragma unroll
   for (int f = 0; f < NF; ++f) pl[f] = e->pl[f];
   opt = e->opt;
@@ -355,9 +364,9 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const I
   int lo = 0, hi = nsrc > 1 ? nsrc : 0;   // one source: nothing to search
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (xi < it.cdf[mid]) hi = mid; else lo = mid + 1;
+    if (xi < global_ptr(it.cdf)[mid]) hi = mid; else lo = mid + 1;
   }
-  const SrcRec &s = it.src[lo < nsrc ? lo : nsrc - 1];
+  const auto &s = global_ptr(it.src)[lo < nsrc ? lo : nsrc - 1];
   p.el = s.mother;
   p.face = s.face;
   const int kind = s.kind;
@@ -614,7 +623,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
   // the lane's photon: its item's element records, tally base and (FD)
   // wave-number factor
-  const Elem<NF, R> *E = BATCH ? nullptr : reinterpret_cast<const Elem<NF, R> *>(P.items[0].elems);
+  ElemPtr<NF, R> E = BATCH ? nullptr : (ElemPtr<NF, R>)(P.items[0].elems);
   int item = 0, tbase = BATCH ? 0 : P.items[0].tbase;
   R kf = BATCH ? R(0) : R(P.items[0].kfac);
 
@@ -678,7 +687,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
             rs.seed((uint32_t)sid, (uint32_t)(sid >> 32), it);
             if constexpr (BATCH) {
               item = pit;
-              E = reinterpret_cast<const Elem<NF, R> *>(it.elems);
+              E = (ElemPtr<NF, R>)(it.elems);
               tbase = it.tbase;
               kf = R(it.kfac);
             }
@@ -876,7 +885,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         }
         // boundary (MC3D.hpp:1536-1567)
         const int ib = -1 - code;
-        const BndRec b = P.items[item].bnd[ib];
+        const auto &bref = global_ptr(P.items[item].bnd)[ib];
+        const struct { double bcn; int32_t kind; } b = {bref.bcn, bref.kind};
         if (b.kind == kBndMirror) {
           R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
           if (D == 3) cd += p.d[2] * nrm[2];
