@@ -336,6 +336,9 @@ void build_light_sources(Model &m) {
       }
     }
   size_t ns = m.lsCDF.size();
+  /* source-type faces that no element references (an inconsistent mesh)
+     leave the list empty; the reference would divide by an empty CDF */
+  if (ns == 0) throw OracleError{E_NO_LIGHTSOURCE};
   for (size_t i = 0; i < ns; ++i) m.lsCDF[i] *= m.BCIntensity[m.lsFace[i]];
   for (size_t i = 1; i < ns; ++i) m.lsCDF[i] += m.lsCDF[i - 1];
   for (size_t i = 0; i < ns; ++i) m.lsCDF[i] /= m.lsCDF[ns - 1];

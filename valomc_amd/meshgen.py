@@ -300,7 +300,8 @@ def smooth_field(cen: np.ndarray, corr_mm: float, seed: int, grid_h: float = 1.0
     F = np.fft.rfftn(noise)
     freqs = [np.fft.fftfreq(s, d=grid_h) for s in shape[:-1]] + [np.fft.rfftfreq(shape[-1], d=grid_h)]
     K2 = sum(np.meshgrid(*[f ** 2 for f in freqs], indexing="ij"))
-    field = np.fft.irfftn(F * np.exp(-2 * (math.pi ** 2) * (corr_mm ** 2) * K2), s=shape)
+    field = np.fft.irfftn(F * np.exp(-2 * (math.pi ** 2) * (corr_mm ** 2) * K2), s=shape,
+                          axes=tuple(range(len(shape))))
     field = (field - field.mean()) / field.std()
     idx = np.clip(np.rint((cen - lo) / grid_h).astype(np.int64), 0, np.array(shape) - 1)
     return field[tuple(idx[:, d] for d in range(cen.shape[1]))]
