@@ -749,28 +749,24 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         const R num = fmax(pl[f].w - q, R(0));
         const bool cur = f == p.face;
         out_cur = cur ? c > Tol<R>::out : out_cur;
-        if (!cur && c > R(0)) {
-          if (num * bc < bn * c) {
-            if constexpr (!kF64) {
-              sn = bn;
-              sc = bc;
-              sf = bf;
-            }
-            bn = num;
-            bc = c;
-            bf = f;
-            code = nb[f];
-            nrm[0] = pl[f].x;
-            nrm[1] = pl[f].y;
-            nrm[2] = D == 3 ? pl[f].z : R(0);
-            mus_n = nbo[f].x;
-            n_n = nbo[f].y;
-          } else if (!kF64 && num * sc < sn * c) {
-            sn = num;
-            sc = c;
-            sf = f;
-          }
+        // selects, not branches: the faces are tested in every lane
+        const bool valid = !cur && c > R(0);
+        const bool best = valid && num * bc < bn * c;
+        if constexpr (!kF64) {
+          const bool second = valid && !best && num * sc < sn * c;
+          sn = best ? bn : (second ? num : sn);
+          sc = best ? bc : (second ? c : sc);
+          sf = best ? bf : (second ? f : sf);
         }
+        bn = best ? num : bn;
+        bc = best ? c : bc;
+        bf = best ? f : bf;
+        code = best ? nb[f] : code;
+        nrm[0] = best ? pl[f].x : nrm[0];
+        nrm[1] = best ? pl[f].y : nrm[1];
+        if (D == 3) nrm[2] = best ? pl[f].z : nrm[2];
+        mus_n = best ? nbo[f].x : mus_n;
+        n_n = best ? nbo[f].y : n_n;
       }
       if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
         // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
