@@ -79,11 +79,13 @@ typedef struct vmc_run_desc {
   int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 6 */
   uint32_t stream_tag;   /* extra stream word: runs with different tags are
                             independent (batch instances use their index)      */
-  int32_t lds_slots;     /* per-workgroup LDS tally cache entries (power of 2);
-                            0 = default (FP32 8192 CW / 4096 FD, FP64 4096 CW
-                            / 2048 FD), -1 = off, at most 16384                */
-  int32_t queue_len;     /* per-wave LDS queue of tally adds drained in batches
-                            (multiple of 64); 0 = default (256), -1 = off      */
+  int32_t lds_slots;     /* per-workgroup LDS tally cache entries (sectors of
+                            8 elements); 0 = default (the LDS the lane buffers
+                            and run queues leave free, about 1400 for FP32 CW),
+                            -1 = off, at most 2^20 (must fit the CU's LDS)   */
+  int32_t queue_len;     /* per-wave LDS queue of sector runs drained with
+                            coalesced atomics; 0 = default (64 CW / 32 FD),
+                            -1 = off, at most 1024                           */
   int32_t reserved[3];
 } vmc_run_desc;
 

@@ -924,6 +924,8 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
   P.queue_len = c.qlen;
   P.weight0 = 0.001;   // MC3D.hpp:182
   P.chance = 0.1;      // MC3D.hpp:183
+  P.weight0_f = (float)P.weight0;
+  P.chance_f = (float)P.chance;
   P.fix_scale = 4294967296.0;
   hip_check(hipEventRecord(a.ev0, a.stream), "event");
   hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, batch, grid, c.block, c.lds, a.stream, P), "walk");

@@ -173,19 +173,24 @@ template <bool FIXED, int WS> struct SectorCache {
   __device__ __forceinline__ int slot_of(int sect) const {
     return (int)(((uint64_t)((uint32_t)sect * 0x9E3779B1u) * (uint32_t)nslots) >> 32);
   }
-  // moves the lane buffer words buf[j * stride] of `sect` into the sector's
-  // slot (clearing them); false, leaving them, when the slot belongs to
+  // the sector's slot, claimed if still free; -1 when the slot belongs to
   // another sector or the cache is off
-  template <class R>
-  __device__ __forceinline__ bool add_from(int sect, R *buf, int stride, double scale) {
-    if (nslots == 0) return false;
+  __device__ __forceinline__ int claim(int sect) {
+    if (nslots == 0) return -1;
     const int sl = slot_of(sect);
     int t = __hip_atomic_load(&tag[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (t == -1) {
       const int old = atomicCAS(&tag[sl], -1, sect);
       t = old == -1 ? sect : old;
     }
-    if (t != sect) return false;
+    return t == sect ? sl : -1;
+  }
+  // moves the lane buffer words buf[j * stride] of `sect` into the sector's
+  // slot (clearing them); false, leaving them, when the sector has no slot
+  template <class R>
+  __device__ __forceinline__ bool add_from(int sect, R *buf, int stride, double scale) {
+    const int sl = claim(sect);
+    if (sl < 0) return false;
 #pragma unroll 4
     for (int j = 0; j < WS; ++j) {
       const R v = buf[j * stride];
@@ -535,7 +540,14 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   __syncthreads();
   uint64_t steps = 0;   // wave-uniform
   int qn = 0;           // wave-uniform: runs in the queue
-  int sect = -1;        // sector held in the lane's buffer
+  // The lane's current sector.  FP64 tallies: a sector with a cache slot
+  // (cslot >= 0) takes each visit's deposits as LDS atomics straight into
+  // the slot; one without sums them in the lane buffer, which goes to the run
+  // queue when the lane moves on.  Fixed point: every sector sums in the
+  // lane buffer, so each rounding covers one photon's run of visits
+  // whichever sectors hold slots.
+  int sect = -1;
+  int cslot = -1;
 
   // Add to the replica.  Word index: element word el*W + w, or boundary
   // word 0x80000000 | ib*4 + j.  With one replica per XCD (`local`), every
@@ -578,7 +590,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   // hand the lane's buffer (sector `sect`) to the cache or the queue if
   // `go`, and clear it; words move LDS to LDS one at a time
   auto flush_sector = [&](bool go) {
-    const bool need = go && !cache.add_from(sect, sbuf, bstride, P.fix_scale);
+    const bool need = go && (!FIXED ? true : !cache.add_from(sect, sbuf, bstride, P.fix_scale));
     uint64_t m = __ballot(need);
     if (m == 0) return;
     if (Q == 0) {
@@ -732,10 +744,11 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         alive = false;
       };
       // exit face: smallest positive plane distance over faces moving outward,
-      // the current face excluded (MC3D.hpp:1020-1056)
-      R bn = R(1), bc = R(0), sn = R(1), sc = R(0);   // best and runner-up distance bn/bc, sn/sc
-      int bf = -1, sf = -1, code = kNbNone;
-      R nrm[3] = {R(0), R(0), R(0)};
+      // the current face excluded (MC3D.hpp:1020-1056); distances num/c are
+      // compared as products, without dividing
+      R bn = R(1), bc = R(0);   // best distance bn/bc
+      R sn = R(1), sc = R(0);   // second best (FP32 only, for the tie test)
+      int bf = -1, code = kNbNone;
       R mus_n = R(0), n_n = R(0);
       bool out_cur = false;
 #pragma unroll
@@ -751,71 +764,84 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         out_cur = cur ? c > Tol<R>::out : out_cur;
         // selects, not branches: the faces are tested in every lane
         const bool valid = !cur && c > R(0);
-        const bool best = valid && num * bc < bn * c;
+        const bool best = valid && (f == 0 || num * bc < bn * c);
         if constexpr (!kF64) {
           const bool second = valid && !best && num * sc < sn * c;
           sn = best ? bn : (second ? num : sn);
           sc = best ? bc : (second ? c : sc);
-          sf = best ? bf : (second ? f : sf);
         }
         bn = best ? num : bn;
         bc = best ? c : bc;
         bf = best ? f : bf;
         code = best ? nb[f] : code;
-        nrm[0] = best ? pl[f].x : nrm[0];
-        nrm[1] = best ? pl[f].y : nrm[1];
-        if (D == 3) nrm[2] = best ? pl[f].z : nrm[2];
         mus_n = best ? nbo[f].x : mus_n;
         n_n = best ? nbo[f].y : n_n;
       }
+      // the record's second half (optical and neighbour words) is used only
+      // past this branch; naming it here keeps its loads issued together
+      // with the planes' instead of after the face test
+      asm volatile("" ::"v"(opt.x), "v"(opt.y), "v"(opt.z), "v"(opt.w), "v"(nb.x), "v"(nb.y), "v"(nb.z));
+      if constexpr (D == 3) asm volatile("" ::"v"(nb.w));
+#pragma unroll
+      for (int f = 0; f < NF; ++f) asm volatile("" ::"v"(nbo[f].x), "v"(nbo[f].y));
       if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
         // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
         lose();
         return;
       }
       if constexpr (!kF64) {
-        // FP32 exit distances within 2^-16 of each other: decide in FP64
-        // from the same record and packet (the record is re-read from the
-        // cache rather than kept live through the loop)
-        if (sf >= 0 && sn * bc - bn * sc < R(1.52587890625e-5) * sn * bc) {
+        // another outward face within 2^-16 of the best distance: decide in
+        // FP64 from the same record and packet (the record is re-read from
+        // the cache rather than kept live through the loop)
+        if (sc > R(0) && sn * bc <= bn * R(1.0000152587890625) * sc) {
           ++nrecheck;
           V4 pl2[NF];
           V4 opt2;
           i4 nb2;
           typename Vec2T<R>::type nbo2[NF];
           load_elem<NF, R>(E, p.el, pl2, opt2, nb2, nbo2);
-          auto t64 = [&](int f) {
-            double c = 0, q = 0;
+          double bt = 0.0;
+          int g = -1;
 #pragma unroll
-            for (int g = 0; g < NF; ++g) {
-              if (g != f) continue;
-              c = (double)pl2[g].x * p.d[0] + (double)pl2[g].y * p.d[1];
-              q = (double)pl2[g].x * p.x[0] + (double)pl2[g].y * p.x[1];
-              if (D == 3) {
-                c += (double)pl2[g].z * p.d[2];
-                q += (double)pl2[g].z * p.x[2];
-              }
-              q = fmax((double)pl2[g].w - q, 0.0);
+          for (int f = 0; f < NF; ++f) {
+            double c = (double)pl2[f].x * p.d[0] + (double)pl2[f].y * p.d[1];
+            double q = (double)pl2[f].x * p.x[0] + (double)pl2[f].y * p.x[1];
+            if (D == 3) {
+              c += (double)pl2[f].z * p.d[2];
+              q += (double)pl2[f].z * p.x[2];
             }
-            return q / c;
-          };
-          if (t64(sf) < t64(bf)) {
-            bn = sn;
-            bc = sc;
-            bf = sf;
+            const double t = fmax((double)pl2[f].w - q, 0.0) / c;
+            const bool take = f != p.face && c > 0.0 && (g < 0 || t < bt);
+            bt = take ? t : bt;
+            g = take ? f : g;
+          }
 #pragma unroll
-            for (int g = 0; g < NF; ++g) {
-              if (g != sf) continue;
-              code = nb2[g];
-              nrm[0] = pl2[g].x;
-              nrm[1] = pl2[g].y;
-              nrm[2] = D == 3 ? pl2[g].z : R(0);
-              mus_n = nbo2[g].x;
-              n_n = nbo2[g].y;
+          for (int f = 0; f < NF; ++f) {
+            if (f != g || g == bf) continue;
+            // the face's FP32 distance, as the first pass computed it
+            R c = pl2[f].x * p.d[0] + pl2[f].y * p.d[1];
+            R q = pl2[f].x * p.x[0] + pl2[f].y * p.x[1];
+            if (D == 3) {
+              c += pl2[f].z * p.d[2];
+              q += pl2[f].z * p.x[2];
             }
+            bn = fmax(pl2[f].w - q, R(0));
+            bc = c;
+            bf = f;
+            code = nb2[f];
+            mus_n = nbo2[f].x;
+            n_n = nbo2[f].y;
           }
         }
       }
+      // outward unit normal of the exit face, for the rarer boundary and
+      // refraction branches (re-read rather than carried through the step)
+      auto exit_normal = [&](R (&nrm)[3]) {
+        const V4 q = (E + p.el)->pl[bf];
+        nrm[0] = q.x;
+        nrm[1] = q.y;
+        nrm[2] = D == 3 ? q.z : R(0);
+      };
       const R dist = w_div(bn, bc);
       const R ds = p.prop < dist ? p.prop : dist;   // fmin(prop, dist), NaN prop -> dist
 #pragma unroll
@@ -883,6 +909,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         const int ib = -1 - code;
         const auto &bref = global_ptr(P.items[item].bnd)[ib];
         const struct { double bcn; int32_t kind; } b = {bref.bcn, bref.kind};
+        R nrm[3];
+        exit_normal(nrm);
         if (b.kind == kBndMirror) {
           R cd = p.d[0] * nrm[0] + p.d[1] * nrm[1];
           if (D == 3) cd += p.d[2] * nrm[2];
@@ -924,21 +952,29 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         // vacuum -> scattering medium: fresh free path
         p.prop = w_neglog(Unif<R>::open(rs.next(P))) / mus_n;
       }
-      if (p.w < R(P.weight0)) {
+      // the roulette constants come as FP32 copies for the FP32 walk, so
+      // they stay in scalar registers
+      const R weight0 = kF64 ? R(P.weight0) : R(P.weight0_f);
+      const R chance = kF64 ? R(P.chance) : R(P.chance_f);
+      if (p.w < weight0) {
         // Russian roulette at the crossing (MC3D.hpp:1597-1604)
-        if (Unif<R>::closed(rs.next(P)) > R(P.chance)) {
+        if (Unif<R>::closed(rs.next(P)) > chance) {
           end_visit();
           atomicAdd(&s_led[wid][1], (double)p.w);
           alive = false;
           return;
         }
-        const R wn = p.w / R(P.chance);
+        const R wn = p.w / chance;
         atomicAdd(&s_led[wid][2], (double)wn - (double)p.w);
         p.w = wn;
       }
-      if (n_c != n_n && fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P)) {
-        p.face = bf;
-        return;
+      if (n_c != n_n) {
+        R nrm[3];
+        exit_normal(nrm);
+        if (fresnel<D, R>(p, nrm, n_c, n_n, true, rs, P)) {
+          p.face = bf;
+          return;
+        }
       }
       if (!(mus_c <= R(0) && mus_n > R(0))) p.prop *= w_div(mus_c, mus_n);
       if (!back) {
@@ -956,11 +992,21 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     // ---- commit the parked tally work (converged) ----
     {
       const int es = pe_el >= 0 ? (tbase + pe_el) / kSectorElems : -1;
-      flush_sector(es >= 0 && sect >= 0 && es != sect);
+      const bool chg = es >= 0 && es != sect;
+      flush_sector(chg && sect >= 0 && cslot < 0);
+      if constexpr (!FIXED) {
+        if (chg) cslot = cache.claim(es);
+      }
       if (es >= 0) {
         const int k = tbase + pe_el - es * kSectorElems;
-        sbuf[(k * W) * bstride] += pe0;
-        if constexpr (FD) sbuf[(k * W + 1) * bstride] += pe1;
+        if (cslot >= 0) {
+          double *cv = reinterpret_cast<double *>(cache.val) + (size_t)cslot * WS + k * W;
+          atomicAdd(cv, (double)pe0);
+          if constexpr (FD) atomicAdd(cv + 1, (double)pe1);
+        } else {
+          sbuf[(k * W) * bstride] += pe0;
+          if constexpr (FD) sbuf[(k * W + 1) * bstride] += pe1;
+        }
         sect = es;
       }
       if constexpr (FIXED) {
@@ -979,7 +1025,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       pb_ib = -1;
     }
   }
-  flush_sector(sect >= 0);
+  flush_sector(sect >= 0 && cslot < 0);
   if (Q > 0) drain();
 
   // ---- flush the counters: one set of global atomics per wave ----

@@ -151,6 +151,7 @@ struct WalkParams {
   int32_t lds_slots;            // LDS sector-cache slots per workgroup (0 = none)
   int32_t queue_len;            // LDS run-queue sectors per wave (0 = direct atomics)
   double weight0, chance;
+  float weight0_f, chance_f;    // the same in FP32, for the FP32 walk
   double fix_scale;             // fixed-point scale (2^32)
 };
 
