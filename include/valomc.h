@@ -81,11 +81,11 @@ typedef struct vmc_run_desc {
                             independent (batch instances use their index)      */
   int32_t lds_slots;     /* per-workgroup LDS tally cache entries (sectors of
                             8 elements); 0 = default (the LDS the lane buffers
-                            and run queues leave free, about 1400 for FP32 CW),
+                            leave free, about 1900 for FP32 CW),
                             -1 = off, at most 2^20 (must fit the CU's LDS)   */
-  int32_t queue_len;     /* per-wave LDS queue of sector runs drained with
-                            coalesced atomics; 0 = default (64 CW / 32 FD),
-                            -1 = off, at most 1024                           */
+  int32_t direct_flush;  /* 0 = default: a lane's sector sums leave LDS as
+                            coalesced atomics, the wave's lanes together;
+                            1 = each lane adds its own words (for testing)   */
   int32_t reserved[3];
 } vmc_run_desc;
 

@@ -119,7 +119,7 @@ def test_fixed_point_tallies_are_bit_reproducible(vmc):
                                   "p2_mirror_hetero"])
 def test_tally_paths_agree_bit_for_bit(vmc, name):
     """Fixed-point adds are exact and order-independent, so the LDS cache,
-    the per-wave LDS queue and direct atomics must give the same bits, for
+    the coalesced flush and per-lane atomics must give the same bits, for
     element words (CW and FD) and every boundary word (3D and 2D)."""
     p = _golden(name)
     p.Nphoton = 50000
@@ -128,20 +128,20 @@ def test_tally_paths_agree_bit_for_bit(vmc, name):
         run = lambda **kw: prob.run(p.Nphoton, seed=11, f=p.f, phase0=p.phase0, precision=vmc.FP32,
                                     tally=vmc.TALLY_FIXED64, **kw)
         base = run()
-        for kw in (dict(lds_slots=-1, queue_len=-1), dict(lds_slots=-1), dict(queue_len=-1),
-                   dict(lds_slots=7, queue_len=1), dict(block_size=64, queue_len=1024)):
+        for kw in (dict(lds_slots=-1, direct_flush=1), dict(lds_slots=-1), dict(direct_flush=1),
+                   dict(lds_slots=7), dict(block_size=64), dict(block_size=64, lds_slots=-1)):
             r = run(**kw)
             for k in OUT:
                 assert np.array_equal(getattr(r, k), getattr(base, k)), (kw, k)
 
 
-def test_tally_queue_rejects_bad_lengths(vmc):
-    """More than 1024 queued sectors per wave is refused."""
+def test_tally_flush_mode_is_validated(vmc):
+    """direct_flush takes 0 or 1 only."""
     p = _golden("p3_pencil_sphere")
     with vmc.Problem(p) as prob:
-        for q in (1025, 8192):
+        for q in (-1, 2, 1024):
             with pytest.raises(vmc.VmcError):
-                prob.run(1000, queue_len=q)
+                prob.run(1000, direct_flush=q)
 
 
 def test_results_do_not_depend_on_launch_shape(vmc):

@@ -53,7 +53,7 @@ class RunDesc(C.Structure):
         ("f", C.c_double), ("phase0", C.c_double), ("Nphoton", C.c_int64), ("seed", C.c_uint64),
         ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
-        ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("queue_len", C.c_int32),
+        ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("direct_flush", C.c_int32),
         ("reserved", C.c_int32 * 3),
     ]
 
@@ -166,14 +166,14 @@ class RunResult:
 def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
                   precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
                   blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-                  lds_slots: int = 0, queue_len: int = 0) -> RunDesc:
+                  lds_slots: int = 0, direct_flush: int = 0) -> RunDesc:
     """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
     rd = RunDesc()
     rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
     rd.precision, rd.tally = int(precision), int(tally)
     rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
     rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
-    rd.queue_len = int(queue_len)
+    rd.direct_flush = int(direct_flush)
     return rd
 
 
@@ -288,12 +288,12 @@ class Problem:
     def run(self, nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-            lds_slots: int = 0, queue_len: int = 0,
+            lds_slots: int = 0, direct_flush: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
-                           queue_len=queue_len)
+                           direct_flush=direct_flush)
         out, o = self._outputs()
         s = Stats()
         cb = None

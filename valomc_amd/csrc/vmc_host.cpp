@@ -814,7 +814,8 @@ const uint64_t kChunk = 128;   // photon ids a wave takes from the dispenser at 
 struct LaunchCfg {
   int dim = 3;
   bool f64 = false, fd = false, fixed = false, xcc = true;
-  int W = 1, block = 256, qlen = 0, slots = 0, refill = 6, nrep = 8;
+  int W = 1, block = 256, slots = 0, refill = 6, nrep = 8;
+  bool direct = false;
   size_t lds = 0;
 };
 
@@ -839,23 +840,22 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   c->xcc = run->replicas <= 0;
   c->nrep = c->xcc ? 8 : run->replicas;
   if (c->nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
-  // run queue (sectors per wave): 0 = default, -1 = off (direct atomics)
+  if (run->direct_flush != 0 && run->direct_flush != 1) return VMC_ERR_INVALID_ARGUMENT;
+  c->direct = run->direct_flush == 1;
   const int rbytes = c->f64 ? 8 : 4;
-  c->qlen = run->queue_len == 0 ? (fd ? 32 : 64) : (run->queue_len < 0 ? 0 : run->queue_len);
-  if (c->qlen > 1024) return VMC_ERR_INVALID_ARGUMENT;
-  // sector cache: 0 = fill the CU's LDS left over by the queues and lane
-  // buffers of the workgroups that fit by registers, -1 = off
+  // sector cache: 0 = fill the CU's LDS left over by the lane buffers of
+  // the workgroups that fit by registers, -1 = off
   c->slots = run->lds_slots < 0 ? 0 : run->lds_slots;
   if (run->lds_slots == 0) {
     const int waves_per_cu = c->f64 ? 8 : 16;
     const int bpc_target = std::max(1, waves_per_cu / (c->block / 64));
     const size_t per_block = (size_t)lds_per_cu / bpc_target;
-    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->qlen, c->block).total + 1024;   // + ledger, alignment
+    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->block).total + 1024;   // + ledger, alignment
     const size_t entry = (size_t)kSectorElems * c->W * 8 + 4;
     c->slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
   }
   if (c->slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
-  c->lds = WalkLds(c->W, rbytes, c->slots, c->qlen, c->block).total;
+  c->lds = WalkLds(c->W, rbytes, c->slots, c->block).total;
   // creating photons for several dead lanes at once keeps the creation
   // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
   c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
@@ -921,7 +921,7 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
   P.chunk = (int32_t)kChunk;
   P.xcc_replicas = c.xcc ? 1 : 0;
   P.lds_slots = c.slots;
-  P.queue_len = c.qlen;
+  P.direct_flush = c.direct ? 1 : 0;
   P.weight0 = 0.001;   // MC3D.hpp:182
   P.chance = 0.1;      // MC3D.hpp:183
   P.weight0_f = (float)P.weight0;

@@ -151,6 +151,18 @@ __device__ __forceinline__ float w_div(float a, float b) { return a * __builtin_
 __device__ __forceinline__ double w_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float w_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ double w_rsqrt(double x) { return 1.0 / sqrt(x); }
+// sqrt(x) of a normal or zero x (no denormal-range scaling in FP32)
+__device__ __forceinline__ float w_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double w_sqrt(double x) { return sqrt(x); }
+// sqrt(x) and 1 / sqrt(x) of x >= 1e-3: one reciprocal square root in FP32
+__device__ __forceinline__ void w_sqrt_pair(float x, float *s, float *is) {
+  *is = __builtin_amdgcn_rsqf(x);
+  *s = x * *is;
+}
+__device__ __forceinline__ void w_sqrt_pair(double x, double *s, double *is) {
+  *s = sqrt(x);
+  *is = 1.0 / *s;
+}
 
 template <class R> struct Tol;
 template <> struct Tol<float> { static constexpr float out = 1e-6f; };
@@ -297,7 +309,7 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkPar
     cost = R(2) * Unif<R>::closed(rs.next(P)) - R(1);
   }
   cost = fmin(fmax(cost, R(-1)), R(1));
-  const R sint = m_sqrt(fmax(R(1) - cost * cost, R(0)));
+  const R sint = w_sqrt(fmax(R(1) - cost * cost, R(0)));
   R sp, cp;
   w_sincos_rev(Unif<R>::closed(rs.next(P)), &sp, &cp);
   R nx, ny, nz;
@@ -306,8 +318,8 @@ __device__ __forceinline__ void scatter3(Photon<R> &p, R g, S &rs, const WalkPar
     ny = sint * sp;
     nz = p.d[2] >= R(0) ? cost : -cost;
   } else {
-    const R tmp = m_sqrt(R(1) - p.d[2] * p.d[2]);
-    const R itmp = w_div(R(1), tmp);
+    R tmp, itmp;   // |dz| <= 0.999 here
+    w_sqrt_pair(R(1) - p.d[2] * p.d[2], &tmp, &itmp);
     nx = sint * (p.d[0] * p.d[2] * cp - p.d[1] * sp) * itmp + p.d[0] * cost;
     ny = sint * (p.d[1] * p.d[2] * cp + p.d[0] * sp) * itmp + p.d[1] * cost;
     nz = -sint * cp * tmp + p.d[2] * cost;
@@ -487,9 +499,10 @@ template <> struct ExitWords<2, true> { static constexpr int n = 4; static const
 // consecutive elements.  Each lane sums its visits into a private LDS
 // buffer for its current sector; when the photon's visits move to another
 // sector, the buffer is flushed as one run of 8W words: into the
-// workgroup's sector cache if the sector holds a slot there, else into the
-// wave's run queue.  A full queue is drained with eight lanes per sector, so
-// each run costs one coalesced atomic request on one 64-byte line.  Device
+// workgroup's sector cache if the sector holds a slot there, else to the
+// replica with eight lanes per line, so each run costs one coalesced atomic
+// request per 64-byte line.  (FP64 tallies add a cached sector's visits to
+// its slot directly.)  Device
 // atomics are served per line request (about 24 G requests/s on MI355X
 // whatever the word type), which makes requests, not words, the budget.
 // BATCH: items of different problems share the launch, so each lane carries
@@ -525,12 +538,9 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   constexpr int WS = kSectorElems * W;   // tally words per sector
   extern __shared__ __align__(16) unsigned char s_dyn[];
   const int S = P.lds_slots > 0 ? P.lds_slots : 0;
-  const int Q = P.queue_len;
-  const WalkLds lay(W, (int)sizeof(R), S, Q, (int)blockDim.x);
+  const WalkLds lay(W, (int)sizeof(R), S, (int)blockDim.x);
   SectorCache<FIXED, WS> cache{reinterpret_cast<int *>(s_dyn + lay.ctag),
                                reinterpret_cast<unsigned long long *>(s_dyn + lay.cval), S};
-  R *const qv = reinterpret_cast<R *>(s_dyn + lay.qval) + (size_t)wid * Q * WS;
-  int *const qrun = reinterpret_cast<int *>(s_dyn + lay.qrun) + (size_t)wid * Q;
   R *const sbuf = reinterpret_cast<R *>(s_dyn + lay.sbuf) + threadIdx.x;   // word j at [j * blockDim]
   const int bstride = (int)blockDim.x;
   for (int i = threadIdx.x; i < S; i += blockDim.x) cache.tag[i] = -1;
@@ -539,11 +549,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   for (int j = 0; j < WS; ++j) sbuf[j * bstride] = R(0);
   __syncthreads();
   uint64_t steps = 0;   // wave-uniform
-  int qn = 0;           // wave-uniform: runs in the queue
   // The lane's current sector.  FP64 tallies: a sector with a cache slot
   // (cslot >= 0) takes each visit's deposits as LDS atomics straight into
-  // the slot; one without sums them in the lane buffer, which goes to the run
-  // queue when the lane moves on.  Fixed point: every sector sums in the
+  // the slot; one without sums them in the lane buffer, which goes to the
+  // replica when the lane moves on.  Fixed point: every sector sums in the
   // lane buffer, so each rounding covers one photon's run of visits
   // whichever sectors hold slots.
   int sect = -1;
@@ -572,28 +581,17 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     if constexpr (FIXED) return (unsigned long long)__double2ll_rn(v * P.fix_scale);
     else return (unsigned long long)__double_as_longlong(v);
   };
-  // converged: every lane of the wave calls these together
-  auto drain = [&]() {
-    __builtin_amdgcn_wave_barrier();
-    constexpr int per = 64 / WS;   // runs per wave instruction
-    const int jr = lane / WS, jw = lane % WS;
-    for (int r0 = 0; r0 < qn; r0 += per) {
-      const int r = r0 + jr;
-      if (r < qn) {
-        const R v = qv[r * WS + jw];
-        if (v != R(0)) global_add((uint32_t)(qrun[r] * WS + jw), to_bits((double)v));
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    qn = 0;
-  };
-  // hand the lane's buffer (sector `sect`) to the cache or the queue if
-  // `go`, and clear it; words move LDS to LDS one at a time
+  // Hands the buffers of the lanes with `go` (sector `sect`) to the cache
+  // (fixed point) or to the replica, and clears them.  Converged: every lane
+  // of the wave calls it together.  The wave moves 64 / (8W) flushing lanes'
+  // buffers per instruction, a group of 8W lanes per buffer and one word per
+  // lane, so a run costs one coalesced atomic request per 64-byte line and
+  // the words go from the lane buffers to the replica without a copy.
   auto flush_sector = [&](bool go) {
     const bool need = go && (!FIXED ? true : !cache.add_from(sect, sbuf, bstride, P.fix_scale));
     uint64_t m = __ballot(need);
     if (m == 0) return;
-    if (Q == 0) {
+    if (P.direct_flush) {
       if (need) {
 #pragma unroll 4
         for (int j = 0; j < WS; ++j) {
@@ -604,25 +602,27 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       }
       return;
     }
+    constexpr int per = 64 / WS;   // buffers per wave instruction
+    const int grp = lane / WS, jw = lane % WS;
+    R *const wave_buf = sbuf - lane;   // word j of the wave's lane l at wave_buf[j * bstride + l]
+    __builtin_amdgcn_wave_barrier();
     while (m) {
-      if (qn == Q) drain();
-      const int room = Q - qn;
-      const bool mine = (m >> lane) & 1ull;
-      const int rk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-      const bool take = mine && rk < room;
-      const uint64_t t = __ballot(take);
-      if (take) {
-        const int r = qn + rk;
-        qrun[r] = sect;
-#pragma unroll 4
-        for (int j = 0; j < WS; ++j) {
-          qv[r * WS + j] = sbuf[j * bstride];
-          sbuf[j * bstride] = R(0);
-        }
+      int src = -1;   // the lane whose buffer this lane's group moves
+#pragma unroll
+      for (int k = 0; k < per; ++k) {
+        const int b = m ? (int)__builtin_ctzll(m) : -1;
+        src = grp == k ? b : src;
+        m &= m - 1;
       }
-      qn += __popcll(t);
-      m &= ~t;
+      const int ss = __shfl(sect, src < 0 ? 0 : src);
+      if (src >= 0) {
+        R *const w = wave_buf + jw * bstride + src;
+        const R v = *w;
+        *w = R(0);
+        if (v != R(0)) global_add((uint32_t)(ss * WS + jw), to_bits((double)v));
+      }
     }
+    __builtin_amdgcn_wave_barrier();
   };
 
   // parked tally work of the last step
@@ -1026,7 +1026,6 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     }
   }
   flush_sector(sect >= 0 && cslot < 0);
-  if (Q > 0) drain();
 
   // ---- flush the counters: one set of global atomics per wave ----
 #pragma unroll

@@ -92,21 +92,17 @@ constexpr int kSectorElems = 8;
 // Dynamic LDS of one walk workgroup, in bytes, in this order (each part
 // 16-byte aligned):
 //   sector cache values  [S][8W]  u64 (FP64 sums or fixed point)
-//   run queue values     [waves][Q][8W]  R
-//   lane sector buffers  [8W][block]     R
+//   lane sector buffers  [8W][block]  R
 //   sector cache tags    [S]  int32
-//   run queue sectors    [waves][Q]  int32
 struct WalkLds {
-  size_t cval, qval, sbuf, ctag, qrun, total;
-  __host__ __device__ WalkLds(int W, int rbytes, int S, int Q, int block) {
-    const size_t ws = (size_t)kSectorElems * W, waves = (size_t)block / 64;
+  size_t cval, sbuf, ctag, total;
+  __host__ __device__ WalkLds(int W, int rbytes, int S, int block) {
+    const size_t ws = (size_t)kSectorElems * W;
     auto up = [](size_t x) { return (x + 15) & ~(size_t)15; };
     cval = 0;
-    qval = up(cval + (size_t)S * ws * 8);
-    sbuf = up(qval + waves * (size_t)Q * ws * rbytes);
+    sbuf = up(cval + (size_t)S * ws * 8);
     ctag = up(sbuf + ws * (size_t)block * rbytes);
-    qrun = up(ctag + (size_t)S * 4);
-    total = up(qrun + waves * (size_t)Q * 4);
+    total = up(ctag + (size_t)S * 4);
   }
 };
 
@@ -149,7 +145,7 @@ struct WalkParams {
   int32_t chunk;
   int32_t xcc_replicas;         // 1: replica = XCD id, workgroup-scope (L2) atomics
   int32_t lds_slots;            // LDS sector-cache slots per workgroup (0 = none)
-  int32_t queue_len;            // LDS run-queue sectors per wave (0 = direct atomics)
+  int32_t direct_flush;         // 1: each lane adds its own sector words (no coalescing)
   double weight0, chance;
   float weight0_f, chance_f;    // the same in FP32, for the FP32 walk
   double fix_scale;             // fixed-point scale (2^32)
