@@ -558,14 +558,12 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   int sect = -1;
   int cslot = -1;
 
-  // Add to the replica.  Word index: element word el*W + w, or boundary
-  // word 0x80000000 | ib*4 + j.  With one replica per XCD (`local`), every
-  // add to a replica comes from the CUs behind one L2, so a workgroup-scope
-  // atomic is exact.  Kernel completion writes the L2s back before the
-  // reduction kernel reads the replicas.
-  auto global_add = [&](uint32_t idx, unsigned long long bits) {
-    void *base = (idx & 0x80000000u) ? btally : etally;
-    const int64_t i = (int64_t)(idx & 0x7fffffffu);
+  // Add to the replica: element word el*W + w (`etally`) or boundary word
+  // ib*4 + j (`btally`).  With one replica per XCD (`local`), every add to a
+  // replica comes from the CUs behind one L2, so a workgroup-scope atomic is
+  // exact.  Kernel completion writes the L2s back before the reduction
+  // kernel reads the replicas.
+  auto replica_add = [&](void *base, int64_t i, unsigned long long bits) {
     if constexpr (FIXED) {
       unsigned long long *a = reinterpret_cast<unsigned long long *>(base) + i;
       if (local) __hip_atomic_fetch_add(a, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -597,7 +595,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         for (int j = 0; j < WS; ++j) {
           const R w = sbuf[j * bstride];
           sbuf[j * bstride] = R(0);
-          if (w != R(0)) global_add((uint32_t)(sect * WS + j), to_bits((double)w));
+          if (w != R(0)) replica_add(etally, sect * WS + j, to_bits((double)w));
         }
       }
       return;
@@ -619,7 +617,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         R *const w = wave_buf + jw * bstride + src;
         const R v = *w;
         *w = R(0);
-        if (v != R(0)) global_add((uint32_t)(ss * WS + jw), to_bits((double)v));
+        if (v != R(0)) replica_add(etally, ss * WS + jw, to_bits((double)v));
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -991,14 +989,15 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
     // ---- commit the parked tally work (converged) ----
     {
-      const int es = pe_el >= 0 ? (tbase + pe_el) / kSectorElems : -1;
+      const int ew = tbase + pe_el;   // arena element slot (>= 0 when pe_el is)
+      const int es = pe_el >= 0 ? (int)((uint32_t)ew / kSectorElems) : -1;
       const bool chg = es >= 0 && es != sect;
       flush_sector(chg && sect >= 0 && cslot < 0);
       if constexpr (!FIXED) {
         if (chg) cslot = cache.claim(es);
       }
       if (es >= 0) {
-        const int k = tbase + pe_el - es * kSectorElems;
+        const int k = (int)((uint32_t)ew % kSectorElems);
         if (cslot >= 0) {
           double *cv = reinterpret_cast<double *>(cache.val) + (size_t)cslot * WS + k * W;
           atomicAdd(cv, (double)pe0);
@@ -1019,7 +1018,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       if (pb_ib >= 0) {
 #pragma unroll
         for (int j = 0; j < XW::n; ++j)
-          global_add(0x80000000u | (uint32_t)(pb_ib * 4 + XW::off[j]), to_bits((double)pb[j]));
+          replica_add(btally, pb_ib * 4 + XW::off[j], to_bits((double)pb[j]));
       }
       pe_el = -1;
       pb_ib = -1;
@@ -1039,7 +1038,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   for (int i = threadIdx.x; i < S * WS; i += blockDim.x) {
     const int sc = cache.tag[i / WS];
     const unsigned long long bits = cache.val[i];
-    if (sc >= 0 && bits) global_add((uint32_t)(sc * WS + i % WS), bits);
+    if (sc >= 0 && bits) replica_add(etally, (int64_t)sc * WS + i % WS, bits);
   }
   if (lane == 0) {
     unsafeAtomicAdd(&P.ledger[0], l_abs);
