@@ -835,8 +835,9 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   // workgroups, two per CU
   c->block = run->block_size > 0 ? run->block_size : (c->f64 ? 256 : 1024);
   if (c->block % 64 != 0 || c->block > (c->f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
-  // replicas == 0: one per XCD, added with L2 atomics; > 0: that many,
-  // picked by workgroup index and added with device-scope atomics
+  // replicas == 0: one per XCD, added with L2 atomics (choose_replicas may
+  // make it a single one for the problem's size); > 0: that many, picked by
+  // workgroup index and added with device-scope atomics
   c->xcc = run->replicas <= 0;
   c->nrep = c->xcc ? 8 : run->replicas;
   if (c->nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
@@ -860,6 +861,21 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
   c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
   return VMC_OK;
+}
+
+// replicas == 0: the tally copies follow the size of one copy.  Up to an
+// XCD's L2 (4 MiB) one copy per XCD, added with L2 atomics, costs nothing
+// extra; a copy that exceeds the L2 but fits the Infinity Cache comfortably
+// (64 MiB) is better kept single, device-scope atomics meeting in the cache
+// instead of eight copies streaming through it (the 2M-tet FD box: +18%);
+// larger ones (many batch instances) go back to one per XCD, whose L2
+// atomics beat device-scope atomics that reach HBM.
+void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb) {
+  if (run->replicas > 0) return;
+  const double bytes = ((double)ne * c.W + (double)nb * 4) * 8;
+  const bool single = bytes > 4.0 * (1 << 20) && bytes <= 64.0 * (1 << 20);
+  c.xcc = !single;
+  c.nrep = single ? 1 : 8;
 }
 
 // Sizes the arena for ne element and nb boundary slots and clears its
@@ -1179,6 +1195,7 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
   try {
     hip_check(hipSetDevice(m->device), "hipSetDevice");
     upload_params(*m, c.f64);
+    choose_replicas(c, run, sector_round(m->Ne), m->Nb);
     arena_reset(m->arena, c, sector_round(m->Ne), m->Nb, 1);
     // photon ranges: one launch, or ~20 when a progress callback wants ticks
     const int nseg = progress ? 20 : 1;
@@ -1257,6 +1274,7 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
           nb += m.Nb;
         }
         if (ne > ((int64_t)1 << 30) / c.W || nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
+        choose_replicas(c, &items[0].run, ne, nb);
         arena_reset(g_batch_arena, c, ne, nb, group.size());
         std::vector<ItemRec> recs;
         for (size_t g = 0; g < group.size(); ++g) {
