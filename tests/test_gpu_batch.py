@@ -140,3 +140,37 @@ def test_variant_without_light_source_is_refused(vmc):
         with pytest.raises(vmc.VmcError) as ei:
             base.variant(BCType=np.full(p.Nb, ord("a"), dtype=np.uint8))
         assert ei.value.status == 1   # 1 + NO_LIGHTSOURCE
+
+
+def test_many_items_equal_single_runs(vmc):
+    """Two dozen items of one problem, with uneven photon counts (one large
+    item, an empty one, a single photon, one just past a dispenser chunk):
+    chunks of many items are in flight at once and items end at different
+    times, yet each item equals its solo run bit for bit (fixed point)."""
+    p = _golden("p3_pencil_sphere")
+    q = _golden("p3_direct_abs_fd")
+    counts = [200000, 0, 1, 129] + [3000 + 1500 * k for k in range(20)]
+    try:
+        prob, probq = vmc.Problem(p), vmc.Problem(q)
+        items = [(prob, dict(nphoton=n, seed=21, stream_tag=k)) for k, n in enumerate(counts)]
+        print(f"batch of {len(items)} CW items", flush=True)
+        res, stats = vmc.batch_run(items, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+        assert stats.photons == sum(counts)
+        for (pr, kw), r in zip(items, res):
+            kw = dict(kw)
+            alone = pr.run(kw.pop("nphoton"), precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
+            assert r.loss == alone.loss
+        itemsq = [(probq, dict(nphoton=n, seed=5, stream_tag=k, f=q.f, phase0=q.phase0))
+                  for k, n in enumerate(counts[4:])]
+        print(f"batch of {len(itemsq)} FD items", flush=True)
+        resq, _ = vmc.batch_run(itemsq, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+        for (pr, kw), r in zip(itemsq, resq):
+            kw = dict(kw)
+            alone = pr.run(kw.pop("nphoton"), precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
+    finally:
+        prob.close()
+        probq.close()

@@ -34,6 +34,10 @@ HIP_FLAGS = [
 ]
 
 
+# host side of the library: preparation, launch engine + C entry points, server
+HOST_SOURCES = ("vmc_prepare.cpp", "vmc_engine.cpp", "vmc_server.cpp")
+
+
 def _run(cmd, cwd=None):
     print("+", " ".join(cmd), flush=True)
     subprocess.check_call(cmd, cwd=cwd)
@@ -44,11 +48,15 @@ def build_library() -> str:
     os.makedirs(objdir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include"), "-I", CSRC]
     k_obj = os.path.join(objdir, "vmc_kernels.o")
-    h_obj = os.path.join(objdir, "vmc_host.o")
     _run([HIPCC, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "vmc_kernels.hip"), "-o", k_obj])
-    _run([HIPCC, *HIP_FLAGS, *inc, "-x", "hip", "-c", os.path.join(CSRC, "vmc_host.cpp"), "-o", h_obj])
+    objs = [k_obj]
+    for name in HOST_SOURCES:
+        obj = os.path.join(objdir, name.replace(".cpp", ".o"))
+        _run([HIPCC, *HIP_FLAGS, *inc, "-x", "hip", "-c", os.path.join(CSRC, name), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", k_obj, h_obj, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
+          f"-Wl,--version-script={os.path.join(CSRC, 'exports.map')}"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,620 @@
+// Launch engine and the problem / batch entry points of the C ABI.
+// vmc_problem_run is the reference's MonteCarlo() (MC3D.hpp:1641-1822): it
+// launches the persistent transport kernel, sums the tally replicas on the
+// device and normalises on the host with the reference's own expressions.
+#include "vmc_host.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+namespace vmc {
+
+// ---------------------------------------------------------------------------
+// Launch engine: persistent walk launches over a list of items (problem
+// instances), shared by single runs and batches
+// ---------------------------------------------------------------------------
+
+
+// Validates a run's precision, tally and tuning fields and resolves the
+// launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of LDS.
+int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c) {
+  (void)num_cus;
+  if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
+  if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
+  c->dim = dim;
+  c->f64 = run->precision == VMC_FP64;
+  c->fixed = run->tally == VMC_TALLY_FIXED64;
+  c->fd = fd;
+  c->W = fd ? 2 : 1;
+  // FP32 transport runs as one 1024-lane workgroup per CU (16 waves sharing
+  // one sector cache), FP64 (occupancy 2 waves/SIMD) as 256-lane
+  // workgroups, two per CU
+  c->block = run->block_size > 0 ? run->block_size : (c->f64 ? 256 : 1024);
+  if (c->block % 64 != 0 || c->block > (c->f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
+  // replicas == 0: one per XCD, added with L2 atomics (choose_replicas may
+  // make it a single one for the problem's size); > 0: that many, picked by
+  // workgroup index and added with device-scope atomics
+  c->xcc = run->replicas <= 0;
+  c->nrep = c->xcc ? 8 : run->replicas;
+  if (c->nrep > 4096) return VMC_ERR_INVALID_ARGUMENT;
+  if (run->direct_flush != 0 && run->direct_flush != 1) return VMC_ERR_INVALID_ARGUMENT;
+  c->direct = run->direct_flush == 1;
+  const int rbytes = c->f64 ? 8 : 4;
+  // sector cache: 0 = fill the CU's LDS left over by the lane buffers of
+  // the workgroups that fit by registers, -1 = off
+  c->slots = run->lds_slots < 0 ? 0 : run->lds_slots;
+  if (run->lds_slots == 0) {
+    const int waves_per_cu = c->f64 ? 8 : 16;
+    const int bpc_target = std::max(1, waves_per_cu / (c->block / 64));
+    const size_t per_block = (size_t)lds_per_cu / bpc_target;
+    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->block).total + 1024;   // + ledger, alignment
+    const size_t entry = (size_t)kSectorElems * c->W * 8 + 4;
+    c->slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
+  }
+  if (c->slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
+  c->lds = WalkLds(c->W, rbytes, c->slots, c->block).total;
+  // creating photons for several dead lanes at once keeps the creation
+  // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
+  c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
+  return VMC_OK;
+}
+
+// replicas == 0: the tally copies follow the size of one copy.  Up to an
+// XCD's L2 (4 MiB) one copy per XCD, added with L2 atomics, costs nothing
+// extra; a copy that exceeds the L2 but fits the Infinity Cache comfortably
+// (64 MiB) is better kept single, device-scope atomics meeting in the cache
+// instead of eight copies streaming through it (the 2M-tet FD box: +18%);
+// larger ones (many batch instances) go back to one per XCD, whose L2
+// atomics beat device-scope atomics that reach HBM.
+void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb) {
+  if (run->replicas > 0) return;
+  const double bytes = ((double)ne * c.W + (double)nb * 4) * 8;
+  const bool single = bytes > 4.0 * (1 << 20) && bytes <= 64.0 * (1 << 20);
+  c.xcc = !single;
+  c.nrep = single ? 1 : 8;
+}
+
+// Sizes the arena for ne element and nb boundary slots and clears its
+// tallies, ledger, counters and per-item loss counts.
+void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems) {
+  a.ne = ne;
+  a.nb = nb;
+  a.W = c.W;
+  a.nrep = c.nrep;
+  a.nitems = nitems;
+  const size_t ew = (size_t)ne * c.W * c.nrep, bw = (size_t)nb * 4 * c.nrep;
+  a.etally.ensure(ew);
+  a.btally.ensure(bw);
+  a.raw_e.ensure((size_t)ne * c.W);
+  a.raw_b.ensure((size_t)nb * 4);
+  a.ledger.ensure(8);
+  a.counters.ensure(8);
+  a.item_loss.ensure(nitems);
+  a.items.ensure(nitems);
+  hip_check(launch_fill_u64(a.etally.p, ew, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.btally.p, bw, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), 8, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.counters.p, 8, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.item_loss.p, nitems, 0ull, a.stream), "fill");
+}
+
+// One persistent launch over `items` (vstart set here, chunk-aligned and in
+// order).  Returns the kernel time from HIP events on the arena's stream and
+// the grid size.
+float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out) {
+  uint64_t v = 0;
+  for (ItemRec &it : items) {
+    it.vstart = v;
+    v += (it.count + kChunk - 1) / kChunk * kChunk;
+  }
+  if (v == 0) return 0.f;
+  const bool batch = items.size() > 1;
+  int bpc = 0;
+  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, batch, c.block, c.lds, &bpc), "occupancy");
+  if (bpc < 1) throw HipFail{hipErrorInvalidConfiguration, "workgroup does not fit on a CU (block size / LDS tally cache)"};
+  const int grid = bpc * num_cus;
+  hip_check(hipMemcpyAsync(a.items.p, items.data(), items.size() * sizeof(ItemRec), hipMemcpyHostToDevice, a.stream),
+            "upload");
+  hip_check(launch_fill_u64(a.counters.p, 1, 0ull, a.stream), "fill");   // dispenser
+  WalkParams P{};
+  P.items = a.items.p;
+  P.nitems = (int32_t)items.size();
+  P.vtotal = v;
+  P.etally = a.etally.p;
+  P.btally = a.btally.p;
+  P.ledger = a.ledger.p;
+  P.counters = a.counters.p;
+  P.item_loss = a.item_loss.p;
+  P.ne = a.ne;
+  P.nb = a.nb;
+  P.nrep = c.nrep;
+  P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
+  P.refill_min = c.refill;
+  P.chunk = (int32_t)kChunk;
+  P.xcc_replicas = c.xcc ? 1 : 0;
+  P.lds_slots = c.slots;
+  P.direct_flush = c.direct ? 1 : 0;
+  P.weight0 = 0.001;   // MC3D.hpp:182
+  P.chance = 0.1;      // MC3D.hpp:183
+  P.weight0_f = (float)P.weight0;
+  P.chance_f = (float)P.chance;
+  P.fix_scale = 4294967296.0;
+  hip_check(hipEventRecord(a.ev0, a.stream), "event");
+  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, batch, grid, c.block, c.lds, a.stream, P), "walk");
+  hip_check(hipEventRecord(a.ev1, a.stream), "event");
+  hip_check(hipEventSynchronize(a.ev1), "sync");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a.ev0, a.ev1), "event");
+  if (grid_out) *grid_out = grid;
+  return ms;
+}
+
+// Sums the replicas and copies raw element words [Ne_pad*W], boundary
+// words [nb*4], the ledger, the counters and the per-item loss counts to
+// the host.
+void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
+                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+  const double inv = c.fixed ? 1.0 / 4294967296.0 : 1.0;
+  const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
+  raw_e.resize(ew);
+  raw_b.resize(bw);
+  loss.resize(a.nitems);
+  hip_check(launch_reduce(c.fixed, a.etally.p, a.raw_e.p, ew, c.nrep, ew, inv, a.stream), "reduce");
+  hip_check(launch_reduce(c.fixed, a.btally.p, a.raw_b.p, bw, c.nrep, bw, inv, a.stream), "reduce");
+  hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, ew * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, bw * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(led, a.ledger.p, 5 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(cnt, a.counters.p, 4 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  if (a.nitems)
+    hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, a.nitems * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipStreamSynchronize(a.stream), "sync");
+}
+
+// Item record of a problem run: its device arrays, stream key and phase.
+ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint64_t count, uint64_t id0,
+                  int64_t tbase, int64_t bbase) {
+  ItemRec it{};
+  it.count = count;
+  it.id0 = id0;
+  it.elems = f64 ? (const void *)m.med->elems64.p : (const void *)m.med->elems32.p;
+  it.cdf = m.cdf.p;
+  it.src = m.dsrc.p;
+  it.bnd = m.dbnd.p;
+  it.tbase = (int32_t)tbase;
+  it.bbase = (int32_t)bbase;
+  it.nsrc = (int32_t)m.src.size();
+  it.key0 = (uint32_t)run->seed;
+  it.key1 = (uint32_t)(run->seed >> 32);
+  it.tag = run->stream_tag;
+  double phase0 = run->phase0;
+  if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
+  it.kfac = 2.0 * M_PI * run->f / kC0;
+  it.cph0 = cos(phase0);
+  it.sph0 = sin(phase0);
+  return it;
+}
+
+// Splits a problem's raw element words (device order, from slot tbase) and
+// boundary words (from slot bbase) and writes normalised outputs
+// (MC3D.hpp:1793-1818).
+void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
+                    vmc_output *out) {
+  const double omega = 2.0 * M_PI * f;
+  const bool fd = omega > 0.0;
+  std::vector<double> ER(m.Ne), EI(m.Ne, 0.0), EBR(m.Nb), EBI(m.Nb), DEBR(m.Nb), DEBI(m.Nb), kn(m.Ne);
+  for (int64_t i = 0; i < m.Ne; ++i) {
+    const int64_t dv = m.inv[i];
+    ER[i] = raw_e[W * dv];
+    if (fd) EI[i] = raw_e[W * dv + 1];
+    kn[i] = omega / kC0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+  }
+  for (int64_t i = 0; i < m.Nb; ++i) {
+    EBR[i] = raw_b[4 * i];
+    EBI[i] = raw_b[4 * i + 1];
+    DEBR[i] = raw_b[4 * i + 2];
+    DEBI[i] = raw_b[4 * i + 3];
+  }
+  normalise(m, omega, kn.data(), N, ER.data(), EI.data(), EBR.data(), EBI.data(), DEBR.data(), DEBI.data());
+  if (!out) return;
+  if (out->ER) std::memcpy(out->ER, ER.data(), 8 * m.Ne);
+  if (out->EI) std::memcpy(out->EI, EI.data(), 8 * m.Ne);
+  if (out->EBR) std::memcpy(out->EBR, EBR.data(), 8 * m.Nb);
+  if (out->EBI) std::memcpy(out->EBI, EBI.data(), 8 * m.Nb);
+  if (out->DEBR) std::memcpy(out->DEBR, DEBR.data(), 8 * m.Nb);
+  if (out->DEBI) std::memcpy(out->DEBI, DEBI.data(), 8 * m.Nb);
+}
+
+int64_t sector_round(int64_t n) { return (n + kSectorElems - 1) / kSectorElems * kSectorElems; }
+
+namespace {
+Arena g_batch_arena;   // tallies of vmc_batch_run (one batch at a time)
+std::mutex g_batch_mutex;
+} // namespace
+
+
+} // namespace vmc
+
+using namespace vmc;
+
+extern "C" {
+
+int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsources) {
+  if (!d) return VMC_ERR_INVALID_ARGUMENT;
+  vmc_problem m;
+  const int st = host_prepare(d, &m);
+  if (st != VMC_OK) return st;
+  if (HN_out) std::memcpy(HN_out, m.HN.data(), sizeof(int64_t) * m.HN.size());
+  if (nsources) *nsources = (int64_t)m.src.size();
+  return VMC_OK;
+}
+
+int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
+  if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  vmc_problem *m = new vmc_problem();
+  const int st = host_prepare(d, m);
+  if (st != VMC_OK) {
+    delete m;
+    return st;
+  }
+  try {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      delete m;
+      g_last_error = "no HIP device";
+      return VMC_ERR_NO_DEVICE;
+    }
+    hip_check(hipGetDevice(&m->device), "hipGetDevice");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, m->device), "hipGetDeviceProperties");
+    m->num_cus = prop.multiProcessorCount;
+    m->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
+                                                             : (int64_t)prop.sharedMemPerBlock;
+    m->arena.open();
+    m->cdf.ensure(m->lsCDF.size());
+    hip_check(hipMemcpy(m->cdf.p, m->lsCDF.data(), m->lsCDF.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    m->dsrc.ensure(m->src.size());
+    hip_check(hipMemcpy(m->dsrc.p, m->src.data(), m->src.size() * sizeof(SrcRec), hipMemcpyHostToDevice), "upload");
+    m->dbnd.ensure(m->bnd.size());
+    hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
+  } catch (HipFail f) {
+    int st = fail_hip(f);
+    vmc_problem_destroy(m);
+    return st;
+  }
+  *out = m;
+  return VMC_OK;
+}
+
+int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *d, vmc_problem **out) {
+  if (!base || !d || !out) return VMC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::shared_ptr<MediumData> med = base->med;
+  if (d->mua || d->mus || d->g || d->n) {
+    med = std::make_shared<MediumData>();
+    med->mua = d->mua ? std::vector<double>(d->mua, d->mua + base->Ne) : base->med->mua;
+    med->mus = d->mus ? std::vector<double>(d->mus, d->mus + base->Ne) : base->med->mus;
+    med->g = d->g ? std::vector<double>(d->g, d->g + base->Ne) : base->med->g;
+    med->n = d->n ? std::vector<double>(d->n, d->n + base->Ne) : base->med->n;
+  }
+  vmc_problem *m = new vmc_problem(base->md, med);
+  m->dim = base->dim;
+  m->nv = base->nv;
+  m->nbv = base->nbv;
+  m->Ne = base->Ne;
+  m->Nb = base->Nb;
+  m->Nr = base->Nr;
+  for (int k = 0; k < 3; ++k) m->shift[k] = base->shift[k];
+  m->device = base->device;
+  m->num_cus = base->num_cus;
+  m->lds_per_cu = base->lds_per_cu;
+  const int64_t Nb = base->Nb;
+  m->BCn = base->BCn;
+  m->BCType = d->BCType ? std::vector<char>(d->BCType, d->BCType + Nb) : base->BCType;
+  m->BCLDirType = d->BCLightDirectionType ? std::vector<char>(d->BCLightDirectionType, d->BCLightDirectionType + Nb)
+                                          : base->BCLDirType;
+  m->BCIntensity = d->BCIntensity ? std::vector<double>(d->BCIntensity, d->BCIntensity + Nb) : base->BCIntensity;
+  m->GaussianSigma = d->GaussianSigma ? std::vector<double>(d->GaussianSigma, d->GaussianSigma + Nb)
+                                      : base->GaussianSigma;
+  m->BCLNormal = d->BCLNormal ? std::vector<double>(d->BCLNormal, d->BCLNormal + Nb * base->dim) : base->BCLNormal_in;
+  m->BCLNormal_in = m->BCLNormal;
+  try {
+    // the checks of ErrorChecks() that a source or medium change can fail
+    bool has_ls = false, has_gauss = false;
+    for (char c : m->BCType) {
+      if (c != 'a') has_ls = true;
+      if (c == 'g') has_gauss = true;
+    }
+    if (!has_ls) throw McError{NO_LIGHTSOURCE};
+    if (m->dim == 2 && has_gauss && m->GaussianSigma.empty()) throw McError{NO_GAUSSIAN_SIGMA};
+    build_light_sources(*m);
+    prepare_directions(*m);
+    build_source_records(*m);
+    build_boundary(*m);
+  } catch (McError e) {
+    delete m;
+    return e.code + 1;
+  }
+  try {
+    hip_check(hipSetDevice(m->device), "hipSetDevice");
+    m->arena.open();
+    m->cdf.ensure(m->lsCDF.size());
+    hip_check(hipMemcpy(m->cdf.p, m->lsCDF.data(), m->lsCDF.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+    m->dsrc.ensure(m->src.size());
+    hip_check(hipMemcpy(m->dsrc.p, m->src.data(), m->src.size() * sizeof(SrcRec), hipMemcpyHostToDevice), "upload");
+    m->dbnd.ensure(m->bnd.size());
+    hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
+  } catch (HipFail f) {
+    int st = fail_hip(f);
+    vmc_problem_destroy(m);
+    return st;
+  }
+  *out = m;
+  return VMC_OK;
+}
+
+int vmc_problem_set_medium(vmc_problem *m, const double *mua, const double *mus, const double *g,
+                           const double *n) {
+  if (!m) return VMC_ERR_INVALID_ARGUMENT;
+  if (m->med.use_count() > 1) {   // shared with a variant: copy on write
+    auto own = std::make_shared<MediumData>();
+    own->mua = m->med->mua;
+    own->mus = m->med->mus;
+    own->g = m->med->g;
+    own->n = m->med->n;
+    m->med = own;
+  }
+  if (mua) m->med->mua.assign(mua, mua + m->Ne);
+  if (mus) m->med->mus.assign(mus, mus + m->Ne);
+  if (g) m->med->g.assign(g, g + m->Ne);
+  if (n) m->med->n.assign(n, n + m->Ne);
+  m->med->dirty32 = m->med->dirty64 = true;
+  return VMC_OK;
+}
+
+int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) {
+  if (!m || !HN) return VMC_ERR_INVALID_ARGUMENT;
+  std::memcpy(HN, m->HN.data(), sizeof(int64_t) * m->HN.size());
+  return VMC_OK;
+}
+
+int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
+                    vmc_progress_fn progress, void *user) {
+  if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  const bool fd = 2.0 * M_PI * run->f > 0.0;
+  LaunchCfg c;
+  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c);
+  if (st != VMC_OK) return st;
+  const int64_t N = run->Nphoton;
+  int aborted = 0, grid = 0;
+  float kernel_ms = 0.f;
+  double led[5] = {0, 0, 0, 0, 0};
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  std::vector<double> raw_e, raw_b;
+  std::vector<unsigned long long> loss;
+  try {
+    hip_check(hipSetDevice(m->device), "hipSetDevice");
+    upload_params(*m, c.f64);
+    choose_replicas(c, run, sector_round(m->Ne), m->Nb);
+    arena_reset(m->arena, c, sector_round(m->Ne), m->Nb, 1);
+    // photon ranges: one launch, or ~20 when a progress callback wants ticks
+    const int nseg = progress ? 20 : 1;
+    int64_t done = 0;
+    for (int sgm = 0; sgm < nseg && done < N; ++sgm) {
+      const int64_t end = (sgm == nseg - 1) ? N : (N * (sgm + 1)) / nseg;
+      if (end <= done) continue;
+      std::vector<ItemRec> items{make_item(*m, run, c.f64, (uint64_t)(end - done), (uint64_t)done, 0, 0)};
+      kernel_ms += launch_items(m->arena, m->num_cus, c, items, &grid);
+      done = end;
+      if (progress && !progress(100.0 * (double)done / (double)(N > 0 ? N : 1), user)) {
+        aborted = 1;
+        break;
+      }
+    }
+    arena_collect(m->arena, c, raw_e, raw_b, led, cnt, loss);
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  // like the reference, an aborted run still divides by the planned N
+  finish_outputs(*m, run->f, N, c.W, raw_e.data(), raw_b.data(), out);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->photons = N;
+    stats->steps = (int64_t)cnt[1];
+    stats->loss = (int64_t)cnt[2];
+    stats->rechecks = (int64_t)cnt[3];
+    stats->kernel_ms = kernel_ms;
+    for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
+    stats->grid_blocks = grid;
+    stats->block_size = c.block;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return aborted ? VMC_ERR_ABORTED : VMC_OK;
+}
+
+int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
+  if (!items || n <= 0) return VMC_ERR_INVALID_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  vmc_problem *m0 = items[0].prob;
+  if (!m0) return VMC_ERR_INVALID_ARGUMENT;
+  // one launch per (dimension, CW/FD) group; precision, tally mode and
+  // tuning come from the first item and must agree across the batch
+  for (int32_t i = 0; i < n; ++i) {
+    const vmc_batch_item &b = items[i];
+    if (!b.prob || b.run.Nphoton < 0 || b.prob->device != m0->device) return VMC_ERR_INVALID_ARGUMENT;
+    if (b.run.precision != items[0].run.precision || b.run.tally != items[0].run.tally)
+      return VMC_ERR_INVALID_ARGUMENT;
+    items[i].loss = 0;
+    items[i].status = VMC_OK;
+  }
+  vmc_stats tot{};
+  float kernel_ms = 0.f;
+  try {
+    hip_check(hipSetDevice(m0->device), "hipSetDevice");
+    std::lock_guard<std::mutex> lock(g_batch_mutex);
+    if (!g_batch_arena.stream) g_batch_arena.open();
+    for (int dim = 2; dim <= 3; ++dim) {
+      for (int fdi = 0; fdi < 2; ++fdi) {
+        std::vector<int32_t> group;
+        for (int32_t i = 0; i < n; ++i)
+          if (items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1))) group.push_back(i);
+        if (group.empty()) continue;
+        LaunchCfg c;
+        int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &c);
+        if (st != VMC_OK) return st;
+        int64_t ne = 0, nb = 0;
+        std::vector<int64_t> tb(group.size()), bb(group.size());
+        for (size_t g = 0; g < group.size(); ++g) {
+          vmc_problem &m = *items[group[g]].prob;
+          upload_params(m, c.f64);
+          tb[g] = ne;
+          bb[g] = nb;
+          ne += sector_round(m.Ne);
+          nb += m.Nb;
+        }
+        if (ne > ((int64_t)1 << 30) / c.W || nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
+        choose_replicas(c, &items[0].run, ne, nb);
+        arena_reset(g_batch_arena, c, ne, nb, group.size());
+        std::vector<ItemRec> recs;
+        for (size_t g = 0; g < group.size(); ++g) {
+          const vmc_batch_item &b = items[group[g]];
+          recs.push_back(make_item(*b.prob, &b.run, c.f64, (uint64_t)b.run.Nphoton, 0, tb[g], bb[g]));
+        }
+        int grid = 0;
+        kernel_ms += launch_items(g_batch_arena, m0->num_cus, c, recs, &grid);
+        std::vector<double> raw_e, raw_b;
+        std::vector<unsigned long long> loss;
+        double led[5];
+        unsigned long long cnt[4];
+        arena_collect(g_batch_arena, c, raw_e, raw_b, led, cnt, loss);
+        for (size_t g = 0; g < group.size(); ++g) {
+          vmc_batch_item &b = items[group[g]];
+          finish_outputs(*b.prob, b.run.f, b.run.Nphoton, c.W, raw_e.data() + tb[g] * c.W, raw_b.data() + bb[g] * 4,
+                         &b.out);
+          b.loss = (int64_t)loss[g];
+          tot.photons += b.run.Nphoton;
+        }
+        tot.steps += (int64_t)cnt[1];
+        tot.loss += (int64_t)cnt[2];
+        tot.rechecks += (int64_t)cnt[3];
+        for (int q = 0; q < 5; ++q) tot.ledger[q] += led[q];
+        tot.grid_blocks = grid;
+        tot.block_size = c.block;
+      }
+    }
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  if (stats) {
+    *stats = tot;
+    stats->kernel_ms = kernel_ms;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return VMC_OK;
+}
+
+void vmc_problem_destroy(vmc_problem *m) {
+  if (!m) return;
+  m->cdf.release();
+  m->dsrc.release();
+  m->dbnd.release();
+  m->arena.close();
+  delete m;
+}
+
+int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *HN,
+                    const int64_t *BH, const double *r, const char *BCType, const double *BCIntensity,
+                    const char *BCLightDirectionType, const double *BCLNormal, const double *BCn,
+                    const double *mua, const double *mus, const double *g, const double *n, double f,
+                    double phase0, int64_t Nphoton, const double *GaussianSigma, uint64_t seed,
+                    int32_t precision, double *ER, double *EI, double *EBR, double *EBI, double *DEBR,
+                    double *DEBI, int64_t *HN_out, int64_t *loss) {
+  vmc_problem_desc d{};
+  d.dim = dim;
+  d.Ne = Ne; d.Nb = Nb; d.Nr = Nr;
+  d.H = H; d.HN = HN; d.BH = BH; d.r = r;
+  d.mua = mua; d.mus = mus; d.g = g; d.n = n;
+  d.BCType = BCType; d.BCn = BCn; d.BCLNormal = BCLNormal;
+  d.BCLightDirectionType = BCLightDirectionType;
+  d.BCIntensity = BCIntensity;
+  d.GaussianSigma = GaussianSigma;
+  vmc_problem *p = nullptr;
+  int st = vmc_problem_create(&d, &p);
+  if (st != VMC_OK) return st;
+  vmc_run_desc rd{};
+  rd.f = f;
+  rd.phase0 = phase0;
+  rd.Nphoton = Nphoton;
+  rd.seed = seed;
+  rd.precision = precision;
+  rd.tally = VMC_TALLY_FP64;
+  vmc_output o{ER, EI, EBR, EBI, DEBR, DEBI};
+  vmc_stats s{};
+  st = vmc_problem_run(p, &rd, &o, &s, nullptr, nullptr);
+  if (st == VMC_OK && HN_out) vmc_problem_get_hn(p, HN_out);
+  if (loss) *loss = s.loss;
+  vmc_problem_destroy(p);
+  return st;
+}
+
+const char *vmc_status_string(int st) {
+  static const char *mc[] = {
+      // cpp/Errors.hpp:24-59 wording
+      "No lightsource",
+      "Scattering anisotropy array is not equal in size to the number of elements",
+      "Absorption array is not equal in size to the number of elements",
+      "Scattering coefficient array is not equal in size to the number of elements",
+      "Refractive index array is not equal in size to the number of elements",
+      "External refractive index array is not equal in size to the number of boundary elements",
+      "No external refractive index array given",
+      "The boundary condition array is not equal in size to the number of boundary elements",
+      "The light direction array is not equal in size to the number of boundary elements",
+      "The light direction type array is not equal in size to the number of boundary elements",
+      "H matrix contain indices that are not in the coordinate matrix",
+      "BH matrix contain indices that are not in the coordinate matrix",
+      "Ill defined mesh - cannot find nodes. Check Mesh.",
+      "Ill defined mesh - some boundary elements are trapped between two elements. A boundary element must belong to a single element. Check Mesh.",
+      "A gaussian light source exists but no sigma parameter was provided",
+      "Ill defined mesh - a complete boundary cannot be constructed. Check Mesh.",
+  };
+  if (st == VMC_OK) return "Success";
+  if (st >= 1 && st <= 16) return mc[st - 1];
+  switch (st) {
+  case VMC_ERR_UNDEFINED_DIRECTION:
+    return "A light source of type L, I or p needs a light direction (type a, r or p)";
+  case VMC_ERR_INVALID_ARGUMENT: return "Invalid argument";
+  case VMC_ERR_HIP: return "HIP runtime error";
+  case VMC_ERR_OUT_OF_MEMORY: return "Out of device memory";
+  case VMC_ERR_NO_DEVICE: return "No GPU device";
+  case VMC_ERR_ABORTED: return "Computation aborted by the progress callback";
+  default: return "Undefined error";
+  }
+}
+
+const char *vmc_last_error(void) { return g_last_error.c_str(); }
+
+int vmc_device_info(char *buf, int32_t buflen, int32_t *num_cus) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return VMC_ERR_NO_DEVICE;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return VMC_ERR_HIP;
+  if (buf && buflen > 0)
+    snprintf(buf, (size_t)buflen, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
+  if (num_cus) *num_cus = prop.multiProcessorCount;
+  return VMC_OK;
+}
+
+const char *vmc_version(void) { return "valomc-mi355x 0.1 (gfx950)"; }
+
+} // extern "C"

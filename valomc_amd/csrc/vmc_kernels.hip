@@ -1085,7 +1085,7 @@ __global__ void reduce_replicas_kernel(const void *rep, double *out, size_t n, i
 } // namespace vmc
 
 // ---------------------------------------------------------------------------
-// Host-side launchers (called from vmc_host.cpp)
+// Host-side launchers (called from the launch engine, vmc_engine.cpp)
 // ---------------------------------------------------------------------------
 
 namespace vmc {

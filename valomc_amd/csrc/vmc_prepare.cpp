@@ -1,0 +1,624 @@
+// ErrorChecks() + the host part of Init() of the reference
+// (cpp/3d/MC3D.hpp:460-738, cpp/2d/MC2D.hpp:390-643): same checks in the
+// same order with the same error codes, the same neighbour table, the same
+// light-source list and CDF, the same BCLNormal preprocessing; then the
+// device element order and the element, source and boundary records the
+// kernels walk on.
+#include "vmc_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace vmc {
+
+thread_local std::string g_last_error;
+
+
+// ---------------------------------------------------------------------------
+// Geometry with the reference's expressions (MC3D.hpp:306-457, MC2D.hpp:303-387)
+// ---------------------------------------------------------------------------
+
+double volume3(const vmc_problem &m, int64_t el) {
+  double ax = m.x(m.h(el, 0), 0), ay = m.x(m.h(el, 0), 1), az = m.x(m.h(el, 0), 2);
+  double bx = m.x(m.h(el, 1), 0), by = m.x(m.h(el, 1), 1), bz = m.x(m.h(el, 1), 2);
+  double cx = m.x(m.h(el, 2), 0), cy = m.x(m.h(el, 2), 1), cz = m.x(m.h(el, 2), 2);
+  double dx = m.x(m.h(el, 3), 0), dy = m.x(m.h(el, 3), 1), dz = m.x(m.h(el, 3), 2);
+  double v = (ax - dx) * ((by - dy) * (cz - dz) - (bz - dz) * (cy - dy)) -
+             (ay - dy) * ((bx - dx) * (cz - dz) - (bz - dz) * (cx - dx)) +
+             (az - dz) * ((bx - dx) * (cy - dy) - (by - dy) * (cx - dx));
+  return fabs(v) / 6.0;
+}
+
+double heron3(const vmc_problem &m, int64_t i0, int64_t i1, int64_t i2) {
+  double a = sqrt(pow(m.x(i1, 0) - m.x(i0, 0), 2) + pow(m.x(i1, 1) - m.x(i0, 1), 2) + pow(m.x(i1, 2) - m.x(i0, 2), 2));
+  double b = sqrt(pow(m.x(i2, 0) - m.x(i0, 0), 2) + pow(m.x(i2, 1) - m.x(i0, 1), 2) + pow(m.x(i2, 2) - m.x(i0, 2), 2));
+  double c = sqrt(pow(m.x(i1, 0) - m.x(i2, 0), 2) + pow(m.x(i1, 1) - m.x(i2, 1), 2) + pow(m.x(i1, 2) - m.x(i2, 2), 2));
+  return sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c)) / 4.0;
+}
+
+double area2(const vmc_problem &m, int64_t el) {
+  double ax = m.x(m.h(el, 1), 0) - m.x(m.h(el, 0), 0);
+  double ay = m.x(m.h(el, 1), 1) - m.x(m.h(el, 0), 1);
+  double bx = m.x(m.h(el, 2), 0) - m.x(m.h(el, 0), 0);
+  double by = m.x(m.h(el, 2), 1) - m.x(m.h(el, 0), 1);
+  double c = ax * by - ay * bx;
+  return 0.5 * (c >= 0.0 ? c : -c);
+}
+
+double len2(const vmc_problem &m, int64_t i0, int64_t i1) {
+  return sqrt(pow(m.x(i1, 0) - m.x(i0, 0), 2) + pow(m.x(i1, 1) - m.x(i0, 1), 2));
+}
+
+void bnd_normal(const vmc_problem &m, int64_t ib, double *nrm) {
+  if (m.dim == 3) {
+    int64_t i0 = m.bh(ib, 0), i1 = m.bh(ib, 1), i2 = m.bh(ib, 2);
+    double x1 = m.x(i1, 0) - m.x(i0, 0), y1 = m.x(i1, 1) - m.x(i0, 1), z1 = m.x(i1, 2) - m.x(i0, 2);
+    double x2 = m.x(i2, 0) - m.x(i0, 0), y2 = m.x(i2, 1) - m.x(i0, 1), z2 = m.x(i2, 2) - m.x(i0, 2);
+    double nx = y1 * z2 - z1 * y2, ny = z1 * x2 - x1 * z2, nz = x1 * y2 - y1 * x2;
+    double len = sqrt(nx * nx + ny * ny + nz * nz);
+    nrm[0] = nx / len; nrm[1] = ny / len; nrm[2] = nz / len;
+  } else {
+    int64_t i0 = m.bh(ib, 0), i1 = m.bh(ib, 1);
+    double dx = m.x(i1, 0) - m.x(i0, 0), dy = m.x(i1, 1) - m.x(i0, 1);
+    double len = sqrt(dx * dx + dy * dy);
+    nrm[0] = dy / len; nrm[1] = -dx / len; nrm[2] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ErrorChecks (MC3D.hpp:460-571, MC2D.hpp:390-513)
+// ---------------------------------------------------------------------------
+
+void error_checks(const vmc_problem &m) {
+  if (m.BCn.empty()) throw McError{MISSING_BCN};
+  bool has_ls = false, has_gauss = false;
+  for (char c : m.BCType) {
+    if (c != 'a') has_ls = true;
+    if (c == 'g') has_gauss = true;
+  }
+  if (!has_ls) throw McError{NO_LIGHTSOURCE};
+  if (m.dim == 2 && has_gauss && m.GaussianSigma.empty()) throw McError{NO_GAUSSIAN_SIGMA};
+  for (int64_t v : m.H)
+    if (v < 0 || v >= m.Nr) throw McError{INCONSISTENT_H};
+  for (int64_t v : m.BH)
+    if (v < 0 || v >= m.Nr) throw McError{INCONSISTENT_BH};
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour table.  Same result as the reference's vertex-incidence search
+// (MC3D.hpp:740-871, MC2D.hpp:645-773): face f of element e gets the other
+// element, or boundary entry -1-ib, holding all of its vertices; two
+// different matches are an inconsistent mesh.  Found here by sorting face
+// keys, O(F log F) instead of O(sum of vertex degree^2).
+// ---------------------------------------------------------------------------
+
+struct FaceKey {
+  int64_t a, b, c;
+  int64_t owner;   // (el << 2 | f) for element faces, -1 - ib for boundary entries
+  bool operator<(const FaceKey &o) const {
+    if (a != o.a) return a < o.a;
+    if (b != o.b) return b < o.b;
+    if (c != o.c) return c < o.c;
+    return owner < o.owner;
+  }
+  bool same(const FaceKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+
+inline void sort3(int64_t &a, int64_t &b, int64_t &c) {
+  if (a > b) std::swap(a, b);
+  if (b > c) std::swap(b, c);
+  if (a > b) std::swap(a, b);
+}
+
+void build_neighbours(vmc_problem &m) {
+  if ((int64_t)m.HN.size() == m.Ne * m.nv) return;   // given (MC3D.hpp:839)
+  m.HN.assign(m.Ne * m.nv, kUnset);
+  std::vector<FaceKey> keys;
+  keys.reserve(m.Ne * m.nv + m.Nb);
+  for (int64_t e = 0; e < m.Ne; ++e)
+    for (int f = 0; f < m.nv; ++f) {
+      FaceKey k;
+      if (m.dim == 3) {
+        k.a = m.h(e, kFace3[f][0]); k.b = m.h(e, kFace3[f][1]); k.c = m.h(e, kFace3[f][2]);
+        sort3(k.a, k.b, k.c);
+      } else {
+        k.a = std::min(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        k.b = std::max(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        k.c = -1;
+      }
+      k.owner = (e << 2) | f;
+      keys.push_back(k);
+    }
+  for (int64_t b = 0; b < m.Nb; ++b) {
+    FaceKey k;
+    if (m.dim == 3) {
+      k.a = m.bh(b, 0); k.b = m.bh(b, 1); k.c = m.bh(b, 2);
+      sort3(k.a, k.b, k.c);
+    } else {
+      k.a = std::min(m.bh(b, 0), m.bh(b, 1));
+      k.b = std::max(m.bh(b, 0), m.bh(b, 1));
+      k.c = -1;
+    }
+    k.owner = -1 - b;
+    keys.push_back(k);
+  }
+  std::sort(keys.begin(), keys.end());
+  const int dup = m.dim == 3 ? INCONSISTENT_MESH_DUPLICATE_NEIGHBORS : INCONSISTENT_MESH;
+  size_t i = 0;
+  while (i < keys.size()) {
+    size_t j = i + 1;
+    while (j < keys.size() && keys[j].same(keys[i])) ++j;
+    // every element face in the group takes the one other entry
+    for (size_t q = i; q < j; ++q) {
+      if (keys[q].owner < 0) continue;
+      const int64_t e = keys[q].owner >> 2;
+      const int f = (int)(keys[q].owner & 3);
+      int64_t val = kUnset;
+      for (size_t o = i; o < j; ++o) {
+        if (o == q) continue;
+        const int64_t cand = keys[o].owner < 0 ? keys[o].owner : (keys[o].owner >> 2);
+        if (cand == e) continue;
+        if (val != kUnset && val != cand) throw McError{dup};
+        val = cand;
+      }
+      m.HN[e + m.Ne * f] = val;
+    }
+    i = j;
+  }
+  if (m.dim == 2)
+    for (int64_t v : m.HN)
+      if (v == kUnset) throw McError{MISSING_BOUNDARY};   // MC2D.hpp:768-771
+}
+
+bool is_source(int dim, char t) {
+  if (dim == 3) return t == 'l' || t == 'L' || t == 'i' || t == 'I' || t == 'c' || t == 'C' || t == 'p';
+  return t == 'l' || t == 'i' || t == 'c' || t == 'g';
+}
+
+// BuildLightSource (MC3D.hpp:941-1002, MC2D.hpp:851-911): element-major order
+void build_light_sources(vmc_problem &m) {
+  m.lsFace.clear(); m.lsMother.clear(); m.lsCDF.clear();
+  for (int64_t e = 0; e < m.Ne; ++e)
+    for (int f = 0; f < m.nv; ++f) {
+      int64_t v = m.hn(e, f);
+      if (v <= -1) {
+        int64_t ib = -1 - v;
+        if (ib < m.Nb && is_source(m.dim, m.BCType[ib])) {
+          m.lsFace.push_back(ib);
+          m.lsMother.push_back(e);
+          m.lsCDF.push_back(m.dim == 3 ? heron3(m, m.bh(ib, 0), m.bh(ib, 1), m.bh(ib, 2))
+                                       : len2(m, m.bh(ib, 0), m.bh(ib, 1)));
+        }
+      }
+    }
+  const size_t ns = m.lsCDF.size();
+  if (ns == 0) throw McError{NO_LIGHTSOURCE};
+  for (size_t i = 0; i < ns; ++i) m.lsCDF[i] *= m.BCIntensity[m.lsFace[i]];
+  for (size_t i = 1; i < ns; ++i) m.lsCDF[i] += m.lsCDF[i - 1];
+  for (size_t i = 0; i < ns; ++i) m.lsCDF[i] /= m.lsCDF[ns - 1];
+}
+
+// BCLNormal preprocessing of Init (MC3D.hpp:670-735, MC2D.hpp:611-640)
+void prepare_directions(vmc_problem &m) {
+  if (m.BCLNormal.empty()) return;
+  if (m.dim == 3) {
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      double len = sqrt(pow(m.bcl(i, 0), 2) + pow(m.bcl(i, 1), 2) + pow(m.bcl(i, 2), 2));
+      if (m.BCLDirType[i] != 'n') continue;   // only 'n' rows are normalised
+      for (int d = 0; d < 3; ++d) m.bcl_ref(i, d) /= len;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      if (m.BCLDirType[i] != 'r') continue;
+      double nn[3], e1[3], e2[3], len, dpn, dp1;
+      bnd_normal(m, i, nn);
+      for (int d = 0; d < 3; ++d) {
+        e1[d] = m.x(m.bh(i, 1), d) - m.x(m.bh(i, 0), d);
+        e2[d] = m.x(m.bh(i, 2), d) - m.x(m.bh(i, 0), d);
+      }
+      dpn = nn[0] * e1[0] + nn[1] * e1[1] + nn[2] * e1[2];
+      for (int d = 0; d < 3; ++d) e1[d] -= dpn * nn[d];
+      len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+      for (int d = 0; d < 3; ++d) e1[d] /= len;
+      dpn = nn[0] * e2[0] + nn[1] * e2[1] + nn[2] * e2[2];
+      dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+      for (int d = 0; d < 3; ++d) e2[d] -= dpn * nn[d] + dp1 * e1[d];
+      len = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+      for (int d = 0; d < 3; ++d) e2[d] /= len;
+      double ox = m.bcl(i, 0), oy = m.bcl(i, 1), oz = m.bcl(i, 2);
+      for (int d = 0; d < 3; ++d) m.bcl_ref(i, d) = ox * e1[d] + oy * e2[d] + oz * nn[d];
+    }
+  } else {
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      double len = sqrt(pow(m.bcl(i, 0), 2) + pow(m.bcl(i, 1), 2));
+      m.bcl_ref(i, 0) /= len;
+      m.bcl_ref(i, 1) /= len;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      if (m.BCLDirType[i] != 'r') continue;
+      double nn[3], t[2], len;
+      bnd_normal(m, i, nn);
+      t[0] = m.x(m.bh(i, 1), 0) - m.x(m.bh(i, 0), 0);
+      t[1] = m.x(m.bh(i, 1), 1) - m.x(m.bh(i, 0), 1);
+      len = sqrt(t[0] * t[0] + t[1] * t[1]);
+      t[0] /= len; t[1] /= len;
+      double ox = m.bcl(i, 0), oy = m.bcl(i, 1);
+      m.bcl_ref(i, 0) = ox * t[0] + oy * nn[0];
+      m.bcl_ref(i, 1) = ox * t[1] + oy * nn[1];
+    }
+  }
+}
+
+// Device order of the elements: sorted along a Morton (Z-order) curve of
+// their centroids, so elements that share faces get nearby ids.  A photon's
+// successive element records and tally words then share cache lines and
+// pages; meshes numbered by construction (a grid generator emits each of the
+// six tetrahedra of a cube in a separate block) otherwise put face
+// neighbours far apart.  Results are mapped back to the caller's numbering.
+uint64_t spread_bits3(uint64_t v) {   // 21 bits -> every third bit
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffull;
+  v = (v | v << 16) & 0x1f0000ff0000ffull;
+  v = (v | v << 8) & 0x100f00f00f00f00full;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+  v = (v | v << 2) & 0x1249249249249249ull;
+  return v;
+}
+uint64_t spread_bits2(uint64_t v) {   // 32 bits -> every second bit
+  v &= 0xffffffffull;
+  v = (v | v << 16) & 0x0000ffff0000ffffull;
+  v = (v | v << 8) & 0x00ff00ff00ff00ffull;
+  v = (v | v << 4) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | v << 2) & 0x3333333333333333ull;
+  v = (v | v << 1) & 0x5555555555555555ull;
+  return v;
+}
+
+void build_order(vmc_problem &m) {
+  const int bits = m.dim == 3 ? 21 : 32;
+  double lo[3] = {0, 0, 0}, maxspan = 0;
+  for (int d = 0; d < m.dim; ++d) {
+    double a = m.x(0, d), b = a;
+    for (int64_t i = 1; i < m.Nr; ++i) {
+      a = std::min(a, m.x(i, d));
+      b = std::max(b, m.x(i, d));
+    }
+    lo[d] = a;
+    maxspan = std::max(maxspan, b - a);
+  }
+  // Morton cells about the size of the hexahedron six tetrahedra (or the
+  // quadrilateral two triangles) of a structured mesh fill, so the elements
+  // of one cell stay consecutive
+  double mean = 0;
+  for (int64_t e = 0; e < m.Ne; ++e) mean += std::fabs(m.dim == 3 ? volume3(m, e) : area2(m, e));
+  mean = m.Ne > 0 ? mean / (double)m.Ne : 0.0;
+  const double cell = m.dim == 3 ? std::cbrt(6.0 * mean) : std::sqrt(2.0 * mean);
+  const double maxcells = (double)((1ull << bits) - 1);
+  const double inv_cell = (cell > 0 && maxspan / cell < maxcells) ? 1.0 / cell
+                                                                  : (maxspan > 0 ? maxcells / maxspan : 1.0);
+  std::vector<uint64_t> key(m.Ne);
+  for (int64_t e = 0; e < m.Ne; ++e) {
+    uint64_t q[3] = {0, 0, 0};
+    for (int d = 0; d < m.dim; ++d) {
+      double c = 0;
+      for (int k = 0; k < m.nv; ++k) c += m.x(m.h(e, k), d);
+      c /= m.nv;
+      q[d] = (uint64_t)std::min(std::max((c - lo[d]) * inv_cell, 0.0), maxcells);
+    }
+    key[e] = m.dim == 3 ? (spread_bits3(q[0]) | spread_bits3(q[1]) << 1 | spread_bits3(q[2]) << 2)
+                        : (spread_bits2(q[0]) | spread_bits2(q[1]) << 1);
+  }
+  m.perm.resize(m.Ne);
+  for (int64_t e = 0; e < m.Ne; ++e) m.perm[e] = (int32_t)e;
+  std::sort(m.perm.begin(), m.perm.end(), [&](int32_t a, int32_t b) {
+    return key[a] != key[b] ? key[a] < key[b] : a < b;
+  });
+  m.inv.resize(m.Ne);
+  for (int64_t d = 0; d < m.Ne; ++d) m.inv[m.perm[d]] = (int32_t)d;
+}
+
+// Per-source launch records (CreatePhoton, MC3D.hpp:1060-1294 / MC2D.hpp:958-1132)
+void build_source_records(vmc_problem &m) {
+  m.src.assign(m.lsFace.size(), SrcRec{});
+  for (size_t i = 0; i < m.lsFace.size(); ++i) {
+    SrcRec &s = m.src[i];
+    const int64_t ib = m.lsFace[i], el = m.lsMother[i];
+    s.ib = (int32_t)ib;
+    s.mother = m.inv[el];
+    s.face = -1;
+    for (int f = 0; f < m.nv; ++f)
+      if (-1 - m.hn(el, f) == ib) { s.face = f; break; }
+    const int nb = m.nbv;
+    double *vv[3] = {s.v0, s.v1, s.v2};
+    for (int c = 0; c < nb; ++c)
+      for (int d = 0; d < m.dim; ++d) vv[c][d] = m.x(m.bh(ib, c), d) - m.shift[d];
+    // inward normal: orient towards the element centroid (MC3D.hpp:1092-1107)
+    double nn[3];
+    bnd_normal(m, ib, nn);
+    double t[3] = {0, 0, 0};
+    for (int d = 0; d < m.dim; ++d) {
+      double cen = 0, fc = 0;
+      for (int c = 0; c < m.nv; ++c) cen += m.x(m.h(el, c), d);
+      for (int c = 0; c < nb; ++c) fc += m.x(m.bh(ib, c), d);
+      t[d] = cen / m.nv - fc / nb;
+    }
+    double dot = 0;
+    for (int d = 0; d < m.dim; ++d) dot += nn[d] * t[d];
+    if (dot < 0.0)
+      for (int d = 0; d < 3; ++d) nn[d] = -nn[d];
+    for (int d = 0; d < 3; ++d) s.nin[d] = nn[d];
+    const char bt = m.BCType[ib];
+    const bool given = m.BCLDirType[ib] != 'n';
+    if (m.dim == 3) {
+      if (!given && (bt == 'L' || bt == 'I' || bt == 'p')) throw McError{-1 + VMC_ERR_UNDEFINED_DIRECTION};
+      if (bt == 'l' || bt == 'L') {
+        s.kind = given ? kSrcGiven : kSrcNormal;
+        if (given)
+          for (int d = 0; d < 3; ++d) s.dir[d] = m.bcl(ib, d);
+      } else if (bt == 'p') {
+        s.kind = kSrcPencil;
+        for (int d = 0; d < 3; ++d) s.dir[d] = m.bcl(ib, d) - m.shift[d];
+      } else if (bt == 'i' || bt == 'I') {
+        s.kind = kSrcIso3;
+      } else {   // 'c', 'C'
+        s.kind = kSrcCos3;
+        double e1[3], e2[3], len, dpn, dp1;
+        for (int d = 0; d < 3; ++d) {
+          e1[d] = m.x(m.bh(ib, 1), d) - m.x(m.bh(ib, 0), d);
+          e2[d] = m.x(m.bh(ib, 2), d) - m.x(m.bh(ib, 0), d);
+        }
+        if (!given) {
+          len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+          for (int d = 0; d < 3; ++d) e1[d] /= len;
+          dpn = nn[0] * e2[0] + nn[1] * e2[1] + nn[2] * e2[2];
+          dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+          for (int d = 0; d < 3; ++d) e2[d] -= dpn * nn[d] + dp1 * e1[d];
+        } else {
+          // frame orthogonal to BCLNormal, first axis still the inward
+          // normal (MC3D.hpp:1259-1285)
+          double b[3] = {m.bcl(ib, 0), m.bcl(ib, 1), m.bcl(ib, 2)};
+          dpn = b[0] * e1[0] + b[1] * e1[1] + b[2] * e1[2];
+          for (int d = 0; d < 3; ++d) e1[d] -= dpn * b[d];
+          len = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+          for (int d = 0; d < 3; ++d) e1[d] /= len;
+          dpn = b[0] * e2[0] + b[1] * e2[1] + b[2] * e2[2];
+          dp1 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+          for (int d = 0; d < 3; ++d) e2[d] -= dpn * b[d] + dp1 * e1[d];
+        }
+        len = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+        for (int d = 0; d < 3; ++d) {
+          e2[d] /= len;
+          s.a1[d] = e1[d];
+          s.a2[d] = e2[d];
+        }
+      }
+    } else {
+      s.phi0 = given ? atan2(m.bcl(ib, 1), m.bcl(ib, 0)) : atan2(nn[1], nn[0]);
+      if (bt == 'l') {
+        s.kind = given ? kSrcGiven : kSrcNormal;
+        if (given) { s.dir[0] = m.bcl(ib, 0); s.dir[1] = m.bcl(ib, 1); }
+      } else if (bt == 'i') {
+        s.kind = kSrcIso2;
+      } else if (bt == 'c') {
+        s.kind = given ? kSrcCos2Rej : kSrcCos2;
+      } else {   // 'g'
+        s.kind = given ? kSrcGauss2Rej : kSrcGauss2;
+        s.gsig = m.GaussianSigma[ib];
+        s.gh = 0.5 - 0.5 * erf(M_PI / s.gsig / 2.0 / sqrt(2.0));
+      }
+    }
+  }
+}
+
+// Element records: outward face planes, neighbour codes, optical parameters.
+template <int NF, class R>
+void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
+  out.assign(m.Ne, Elem<NF, R>{});
+  for (int64_t dev = 0; dev < m.Ne; ++dev) {
+    const int64_t e = m.perm[dev];
+    double v[4][3] = {{0}};
+    for (int c = 0; c < m.nv; ++c)
+      for (int d = 0; d < m.dim; ++d) v[c][d] = m.x(m.h(e, c), d) - m.shift[d];
+    Elem<NF, R> &E = out[dev];
+    for (int f = 0; f < NF; ++f) {
+      double nrm[3] = {0, 0, 0}, off = 0;
+      if (m.dim == 3) {
+        const double *a = v[kFace3[f][0]], *b = v[kFace3[f][1]], *c = v[kFace3[f][2]];
+        const double *o = v[6 - kFace3[f][0] - kFace3[f][1] - kFace3[f][2]];
+        double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        double w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        nrm[0] = u[1] * w[2] - u[2] * w[1];
+        nrm[1] = u[2] * w[0] - u[0] * w[2];
+        nrm[2] = u[0] * w[1] - u[1] * w[0];
+        double len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+        for (int d = 0; d < 3; ++d) nrm[d] /= len;
+        if (nrm[0] * (o[0] - a[0]) + nrm[1] * (o[1] - a[1]) + nrm[2] * (o[2] - a[2]) > 0)
+          for (int d = 0; d < 3; ++d) nrm[d] = -nrm[d];
+        for (int d = 0; d < 3; ++d) off += nrm[d] * (a[d] + b[d] + c[d]) / 3.0;
+      } else {
+        const double *a = v[kEdge2[f][0]], *b = v[kEdge2[f][1]];
+        const double *o = v[3 - kEdge2[f][0] - kEdge2[f][1]];
+        double dx = b[0] - a[0], dy = b[1] - a[1];
+        double len = sqrt(dx * dx + dy * dy);
+        nrm[0] = dy / len;
+        nrm[1] = -dx / len;
+        if (nrm[0] * (o[0] - a[0]) + nrm[1] * (o[1] - a[1]) > 0) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; }
+        off = nrm[0] * (a[0] + b[0]) / 2.0 + nrm[1] * (a[1] + b[1]) / 2.0;
+      }
+      E.pl[f].x = (R)nrm[0];
+      E.pl[f].y = (R)nrm[1];
+      E.pl[f].z = (R)nrm[2];
+      E.pl[f].w = (R)off;
+    }
+    E.opt.x = (R)m.med->mua[e];
+    E.opt.y = (R)m.med->mus[e];
+    E.opt.z = (R)m.med->n[e];
+    E.opt.w = (R)m.med->g[e];
+    E.nb = m.nbcode[e];
+    for (int f = 0; f < NF; ++f) {
+      const int code = m.nbcode[e][f];
+      const int64_t nx = code >= 0 ? (int64_t)m.perm[code >> 3] : -1;
+      E.nbo[f].x = nx >= 0 ? (R)m.med->mus[nx] : R(0);
+      E.nbo[f].y = nx >= 0 ? (R)m.med->n[nx] : R(0);
+    }
+  }
+}
+
+void build_nb_codes(vmc_problem &m) {
+  m.nbcode.assign(m.Ne, i4{kNbNone, kNbNone, kNbNone, kNbNone});
+  for (int64_t e = 0; e < m.Ne; ++e) {
+    int code[4] = {kNbNone, kNbNone, kNbNone, kNbNone};
+    for (int f = 0; f < m.nv; ++f) {
+      const int64_t v = m.hn(e, f);
+      if (v == kUnset) continue;
+      if (v < 0) {
+        if (-1 - v < m.Nb) code[f] = (int)v;
+        continue;
+      }
+      if (v >= m.Ne) continue;
+      int entry = 0, back = 0;
+      for (int q = 0; q < m.nv; ++q)
+        if (m.hn(v, q) == e) { entry = q; back = 1; break; }   // MC3D.hpp:1616-1628
+      code[f] = (int)(((int64_t)m.inv[v] << 3) | (back << 2) | entry);
+    }
+    m.nbcode[e] = i4{code[0], code[1], code[2], code[3]};
+  }
+}
+
+void build_boundary(vmc_problem &m) {
+  m.bnd.assign(m.Nb, BndRec{});
+  for (int64_t i = 0; i < m.Nb; ++i) {
+    const char t = m.BCType[i];
+    BndRec &b = m.bnd[i];
+    b.bcn = m.BCn[i];
+    if (t == 'm' || t == 'L' || t == 'I' || t == 'C') b.kind = kBndMirror;   // MC3D.hpp:1541
+    else b.kind = m.BCn[i] > 0.0 ? kBndFresnel : kBndAbsorb;                  // MC3D.hpp:1552
+  }
+}
+
+void upload_params(vmc_problem &m, bool f64) {
+  if (f64 && !m.med->dirty64) return;
+  if (!f64 && !m.med->dirty32) return;
+  if (m.dim == 3) {
+    if (f64) {
+      std::vector<Elem<4, double>> v;
+      pack_elements<4, double>(m, v);
+      m.med->elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    } else {
+      std::vector<Elem<4, float>> v;
+      pack_elements<4, float>(m, v);
+      m.med->elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    }
+  } else {
+    if (f64) {
+      std::vector<Elem<3, double>> v;
+      pack_elements<3, double>(m, v);
+      m.med->elems64.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems64.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    } else {
+      std::vector<Elem<3, float>> v;
+      pack_elements<3, float>(m, v);
+      m.med->elems32.ensure(v.size() * sizeof(v[0]));
+      hip_check(hipMemcpy(m.med->elems32.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice), "upload");
+    }
+  }
+  (f64 ? m.med->dirty64 : m.med->dirty32) = false;
+}
+
+// Normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703)
+void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t N, double *ER, double *EI,
+               double *EBR, double *EBI, double *DEBR, double *DEBI) {
+  const bool d3 = m.dim == 3;
+  if (omega <= 0.0) {
+    for (int64_t i = 0; i < m.Ne; ++i) {
+      if (m.med->mua[i] > 0.0) ER[i] /= m.med->mua[i] * m.vol[i] * (double)N;
+      else ER[i] /= m.vol[i] * (double)N;
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) EBR[i] /= (double)N * m.area[i];
+    if (!d3)
+      for (int64_t i = 0; i < m.Nb; ++i) DEBR[i] /= (double)N * m.area[i];
+  } else {
+    for (int64_t i = 0; i < m.Ne; ++i) {
+      const double a = ER[i], b = EI[i], k = kfac_n[i];
+      ER[i] = (b * k + a * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
+      EI[i] = -(a * k - b * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
+    }
+    for (int64_t i = 0; i < m.Nb; ++i) {
+      EBR[i] /= (double)N * m.area[i];
+      EBI[i] /= (double)N * m.area[i];
+      if (!d3) {
+        DEBR[i] /= (double)N * m.area[i];
+        DEBI[i] /= (double)N * m.area[i];
+      }
+    }
+  }
+}
+
+int fail_hip(const HipFail &f) {
+  g_last_error = std::string(f.what) + ": " + hipGetErrorString(f.e);
+  return f.e == hipErrorOutOfMemory ? VMC_ERR_OUT_OF_MEMORY : VMC_ERR_HIP;
+}
+
+
+
+// ErrorChecks() and the host part of Init(); returns a status.
+int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
+  if ((d->dim != 2 && d->dim != 3) || d->Ne <= 0 || d->Nb <= 0 || d->Nr <= 0 || !d->H || !d->BH || !d->r ||
+      !d->mua || !d->mus || !d->g || !d->n || !d->BCType)
+    return VMC_ERR_INVALID_ARGUMENT;
+  if (d->Ne >= kMaxElements || d->Nb >= INT32_MAX) return VMC_ERR_INVALID_ARGUMENT;
+  try {
+    m->dim = d->dim;
+    m->nv = d->dim + 1;
+    m->nbv = d->dim;
+    m->Ne = d->Ne; m->Nb = d->Nb; m->Nr = d->Nr;
+    copy_in(m->H, d->H, d->Ne * m->nv);
+    copy_in(m->HN, d->HN, d->Ne * m->nv);
+    copy_in(m->BH, d->BH, d->Nb * m->nbv);
+    copy_in(m->r, d->r, d->Nr * m->dim);
+    copy_in(m->med->mua, d->mua, d->Ne);
+    copy_in(m->med->mus, d->mus, d->Ne);
+    copy_in(m->med->g, d->g, d->Ne);
+    copy_in(m->med->n, d->n, d->Ne);
+    copy_in(m->BCType, d->BCType, d->Nb);
+    copy_in(m->BCn, d->BCn, d->Nb);
+    copy_in(m->BCLNormal, d->BCLNormal, d->Nb * m->dim);
+    copy_in(m->BCLDirType, d->BCLightDirectionType, d->Nb);
+    copy_in(m->BCIntensity, d->BCIntensity, d->Nb);
+    if (m->dim == 2) copy_in(m->GaussianSigma, d->GaussianSigma, d->Nb);
+
+    error_checks(*m);
+    build_neighbours(*m);
+    if (m->BCIntensity.empty()) m->BCIntensity.assign(m->Nb, 1.0);
+    if (m->BCLDirType.empty()) m->BCLDirType.assign(m->Nb, 'n');
+    build_light_sources(*m);
+    m->BCLNormal_in = m->BCLNormal;
+    prepare_directions(*m);
+
+    for (int dd = 0; dd < m->dim; ++dd) {
+      double lo = m->x(0, dd), hi = lo;
+      for (int64_t i = 1; i < m->Nr; ++i) {
+        lo = std::min(lo, m->x(i, dd));
+        hi = std::max(hi, m->x(i, dd));
+      }
+      m->shift[dd] = 0.5 * (lo + hi);
+    }
+    build_order(*m);
+    build_source_records(*m);
+    build_nb_codes(*m);
+    build_boundary(*m);
+    m->vol.resize(m->Ne);
+    m->area.resize(m->Nb);
+    for (int64_t i = 0; i < m->Ne; ++i) m->vol[i] = m->dim == 3 ? volume3(*m, i) : area2(*m, i);
+    for (int64_t i = 0; i < m->Nb; ++i)
+      m->area[i] = m->dim == 3 ? heron3(*m, m->bh(i, 0), m->bh(i, 1), m->bh(i, 2)) : len2(*m, m->bh(i, 0), m->bh(i, 1));
+  } catch (McError e) {
+    return e.code + 1;
+  }
+  return VMC_OK;
+}
+
+
+} // namespace vmc
