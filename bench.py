@@ -60,6 +60,10 @@ def parse():
                          "(64 pencil positions x 4 parameter sets) of the same box in one launch")
     ap.add_argument("--photon-scale", type=float, default=0.1,
                     help="batch workload: photon counts are log-uniform in [1e6, 1e8] times this")
+    ap.add_argument("--autotune", action="store_true",
+                    help="box/fd/square2d: pick block size, refill threshold, photons per wave fetch and "
+                         "tally strategy with vmc_problem_autotune before the timed steps (tuning time "
+                         "is reported, not timed)")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="only run a streaming copy kernel (out = -in, float32) over this many bytes "
                          "three times: the reference dispatch that scales FETCH_SIZE / WRITE_SIZE")
@@ -285,6 +289,12 @@ def main():
               replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64,
               f=float(getattr(p, "f", 0.0) or 0.0))
 
+    tuned = None
+    if a.autotune:
+        t = time.perf_counter()
+        tuned = prob.autotune(nphot, precision=vmc.FP32, **kw)
+        tuned["tuning_s"] = round(time.perf_counter() - t, 3)
+        kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots")})
     for i in range(a.warmup):
         prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)
     runs = []
@@ -344,6 +354,8 @@ def main():
         "grid": [runs[-1].grid_blocks, runs[-1].block_size],
         **extra,
     }
+    if tuned is not None:
+        line["autotune"] = tuned
     print(json.dumps(line), flush=True)
 
 

@@ -88,7 +88,9 @@ typedef struct vmc_run_desc {
   int32_t direct_flush;  /* 0 = default: a lane's sector sums leave LDS as
                             coalesced atomics, the wave's lanes together;
                             1 = each lane adds its own words (for testing)   */
-  int32_t reserved[3];
+  int32_t chunk;         /* photon ids a wave takes from the dispenser at a
+                            time: 0 = 128, else a power of 2 in [64, 4096]   */
+  int32_t reserved[2];
 } vmc_run_desc;
 
 typedef struct vmc_output {
@@ -126,6 +128,18 @@ int vmc_problem_create(const vmc_problem_desc *desc, vmc_problem **out);
    Nphoton; the status is then VMC_ERR_ABORTED. */
 int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
                     vmc_stats *stats, vmc_progress_fn progress, void *user);
+
+/* Autotuner for a problem's launches (no counterpart in the reference).
+   Times short runs of *run on prob -- trial_photons photons each, 0 = 2 %
+   of run->Nphoton within [1e5, 1e7] -- over the launch shape (block_size,
+   refill_min, chunk: photons per wave fetch) and the tally strategy
+   (lds_slots cache on/off, replicas), one knob at a time from the defaults,
+   and writes *run with the fastest values into *tuned.  A knob moves only
+   for a gain above 2 %.  Results do not depend on these knobs (bit for bit
+   with VMC_TALLY_FIXED64).  trials (optional) receives the number of timed
+   runs. */
+int vmc_problem_autotune(vmc_problem *prob, const vmc_run_desc *run, vmc_run_desc *tuned,
+                         int64_t trial_photons, int32_t *trials);
 
 /* Host-only ErrorChecks() + the host part of Init(): validation, neighbour
    table and light-source list, without touching a device.  Replaces the

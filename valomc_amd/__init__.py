@@ -54,7 +54,7 @@ class RunDesc(C.Structure):
         ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
         ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("direct_flush", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("chunk", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -105,6 +105,8 @@ def load_library():
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
+        lib.vmc_problem_autotune.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.POINTER(RunDesc), C.c_int64,
+                                             C.POINTER(C.c_int32)]
         lib.vmc_problem_create_variant.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
         lib.vmc_server_create.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         lib.vmc_server_submit.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
@@ -166,14 +168,14 @@ class RunResult:
 def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
                   precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
                   blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-                  lds_slots: int = 0, direct_flush: int = 0) -> RunDesc:
+                  lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0) -> RunDesc:
     """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
     rd = RunDesc()
     rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
     rd.precision, rd.tally = int(precision), int(tally)
     rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
     rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
-    rd.direct_flush = int(direct_flush)
+    rd.direct_flush, rd.chunk = int(direct_flush), int(chunk)
     return rd
 
 
@@ -288,12 +290,12 @@ class Problem:
     def run(self, nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-            lds_slots: int = 0, direct_flush: int = 0,
+            lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
-                           direct_flush=direct_flush)
+                           direct_flush=direct_flush, chunk=chunk)
         out, o = self._outputs()
         s = Stats()
         cb = None
@@ -307,6 +309,21 @@ class Problem:
                          total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)),
                          photons=int(s.photons), grid_blocks=int(s.grid_blocks),
                          block_size=int(s.block_size), rechecks=int(s.rechecks), **out)
+
+    def autotune(self, nphoton: int, trial_photons: int = 0, **run_kwargs) -> dict:
+        """vmc_problem_autotune: the launch shape and tally strategy that run
+        ``run(nphoton, **run_kwargs)`` fastest, as keyword arguments for
+        ``run`` (block_size, refill_min, chunk, replicas, lds_slots), plus
+        ``trials``, the number of timed trial runs."""
+        rd = make_run_desc(nphoton, **run_kwargs)
+        tuned = RunDesc()
+        n = C.c_int32(0)
+        st = self.lib.vmc_problem_autotune(self._h, C.byref(rd), C.byref(tuned), C.c_int64(int(trial_photons)),
+                                           C.byref(n))
+        if st:
+            raise VmcError(st, self.lib)
+        return dict(block_size=tuned.block_size, refill_min=tuned.refill_min, chunk=tuned.chunk,
+                    replicas=tuned.replicas, lds_slots=tuned.lds_slots, trials=int(n.value))
 
 
 def batch_run(items, **common):

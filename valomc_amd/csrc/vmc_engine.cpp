@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 
@@ -61,6 +62,11 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   // creating photons for several dead lanes at once keeps the creation
   // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
   c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
+  // photon ids per dispenser fetch: a power of 2 (chunks never straddle
+  // items), at least a wave's 64 lanes (one fetch refills all dead lanes)
+  if (run->chunk != 0 && (run->chunk < 64 || run->chunk > 4096 || (run->chunk & (run->chunk - 1))))
+    return VMC_ERR_INVALID_ARGUMENT;
+  c->chunk = run->chunk ? (uint64_t)run->chunk : kChunk;
   return VMC_OK;
 }
 
@@ -110,7 +116,7 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
   uint64_t v = 0;
   for (ItemRec &it : items) {
     it.vstart = v;
-    v += (it.count + kChunk - 1) / kChunk * kChunk;
+    v += (it.count + c.chunk - 1) / c.chunk * c.chunk;
   }
   if (v == 0) return 0.f;
   const bool batch = items.size() > 1;
@@ -135,7 +141,7 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
   P.nrep = c.nrep;
   P.step_cap = 1u << 22;   // a photon still walking after 4M steps is counted as lost
   P.refill_min = c.refill;
-  P.chunk = (int32_t)kChunk;
+  P.chunk = (int32_t)c.chunk;
   P.xcc_replicas = c.xcc ? 1 : 0;
   P.lds_slots = c.slots;
   P.direct_flush = c.direct ? 1 : 0;
@@ -437,6 +443,64 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
   return aborted ? VMC_ERR_ABORTED : VMC_OK;
+}
+
+int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *tuned, int64_t trial_photons,
+                         int32_t *trials) {
+  if (!m || !run || !tuned || run->Nphoton < 0 || trial_photons < 0) return VMC_ERR_INVALID_ARGUMENT;
+  const int64_t n = trial_photons > 0 ? trial_photons
+                                      : std::min<int64_t>(10000000, std::max<int64_t>(100000, run->Nphoton / 50));
+  vmc_output none{};
+  int count = 0;
+  // kernel time of a trial; +inf for a configuration the device refuses
+  auto time_of = [&](vmc_run_desc cand) {
+    cand.Nphoton = n;
+    vmc_stats st{};
+    ++count;
+    const int s = vmc_problem_run(m, &cand, &none, &st, nullptr, nullptr);
+    return s == VMC_OK ? st.kernel_ms : std::numeric_limits<double>::infinity();
+  };
+  vmc_run_desc best = *run;
+  {
+    LaunchCfg c;
+    const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, 2.0 * M_PI * run->f > 0.0, run, &c);
+    if (st != VMC_OK) return st;
+  }
+  // first run of a configuration allocates the arena and loads the code
+  if (!std::isfinite(time_of(best))) return VMC_ERR_INVALID_ARGUMENT;
+  const bool f64 = run->precision == VMC_FP64;
+  // defaults spelled out, so a knob's default value is not timed twice
+  if (best.block_size == 0) best.block_size = f64 ? 256 : 1024;
+  if (best.refill_min == 0) best.refill_min = 6;
+  if (best.chunk == 0) best.chunk = (int32_t)kChunk;
+  double tb = time_of(best);
+  struct Knob {
+    int32_t vmc_run_desc::*field;
+    std::vector<int32_t> values;
+  };
+  const std::vector<Knob> knobs = {
+      {&vmc_run_desc::block_size, f64 ? std::vector<int32_t>{256, 128, 64} : std::vector<int32_t>{1024, 512, 256}},
+      {&vmc_run_desc::refill_min, {6, 2, 16, 32}},
+      {&vmc_run_desc::chunk, {128, 64, 256, 512}},
+      {&vmc_run_desc::replicas, {0, 8, 1}},
+      {&vmc_run_desc::lds_slots, {0, -1}},
+  };
+  for (const Knob &k : knobs) {
+    for (int32_t v : k.values) {
+      if (best.*(k.field) == v) continue;
+      vmc_run_desc cand = best;
+      cand.*(k.field) = v;
+      const double t = time_of(cand);
+      if (t < 0.98 * tb) {
+        best = cand;
+        tb = t;
+      }
+    }
+  }
+  *tuned = best;
+  tuned->Nphoton = run->Nphoton;
+  if (trials) *trials = count;
+  return VMC_OK;
 }
 
 int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {

@@ -202,13 +202,14 @@ template <class T> void copy_in(std::vector<T> &dst, const T *src, int64_t n) {
 
 // ---- launch engine (vmc_engine.cpp) ----
 
-constexpr uint64_t kChunk = 128;   // photon ids a wave takes from the dispenser at once
+constexpr uint64_t kChunk = 128;   // default photon ids a wave takes from the dispenser at once
 
 struct LaunchCfg {
   int dim = 3;
   bool f64 = false, fd = false, fixed = false, xcc = true;
   int W = 1, block = 256, slots = 0, refill = 6, nrep = 8;
   bool direct = false;
+  uint64_t chunk = kChunk;
   size_t lds = 0;
 };
 
