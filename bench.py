@@ -138,7 +138,8 @@ def cpu_baseline(p, target_s: float, what: str = "the same problem"):
 def run_batch(a):
     """256 forward solves on the 384k-tet box in one persistent launch
     (FP32 transport, separate FP64 tallies per instance), against the same
-    items run one vmc_problem_run at a time."""
+    items run one vmc_problem_run at a time, and replayed from a batch plan
+    (the launch recorded once as a HIP graph)."""
     import valomc_amd as vmc
     from valomc_amd import meshgen
     base_p, sources, media, photons = meshgen.batched_box_instances(seed=1234, photon_scale=a.photon_scale)
@@ -169,6 +170,17 @@ def run_batch(a):
         seq.append(r)
     seq_wall = time.perf_counter() - t0
     same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, seq))
+    # the same batch recorded once as a HIP graph and replayed (new seeds on
+    # the first replay, then the timed seeds)
+    t0 = time.perf_counter()
+    plan = vmc.BatchPlan(items, **kw)
+    plan_s = time.perf_counter() - t0
+    plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
+    t0 = time.perf_counter()
+    gres, gst = plan.run()
+    graph_wall = time.perf_counter() - t0
+    plan.close()
+    graph_same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(gres, res))
     # accuracy: FP32 batch items against FP64 runs of the same streams (first 4)
     rel = []
     for (pr, k), r in list(zip(items, res))[:4]:
@@ -187,6 +199,10 @@ def run_batch(a):
         "sequential_photons_per_s": round(total / seq_wall, 1),
         "sequential_kernel_ms": round(seq_kernel, 3),
         "batch_vs_sequential_rel_l2_max": same,
+        "graph_replay_photons_per_s": round(total / graph_wall, 1),
+        "graph_replay_kernel_ms": round(gst.kernel_ms, 3),
+        "graph_plan_create_s": round(plan_s, 3),
+        "graph_vs_eager_rel_l2_max": graph_same,
         "fp32_vs_fp64_rel_l2_first4": rel,
     }
     print(json.dumps(line), flush=True)

@@ -200,6 +200,24 @@ typedef struct vmc_batch_item {
    sums the whole batch. */
 int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats);
 
+/* A batch recorded once as a HIP graph and replayed; no counterpart in the
+   reference.  For loops that run the same batch again and again with new
+   seeds or media (an inverse solver's iterations).  Creation checks the
+   items as vmc_batch_run does, allocates every device buffer, and records
+   the batch's device work in one graph: clearing of the tallies, the
+   persistent walk launch of each (dimension, CW/FD) group and the replica
+   sums.  The launch shape and tally strategy come from items[0].run.
+   A run uploads the items' current records (seed, stream_tag, f and phase0
+   with the same CW/FD split, and the problems' current media), replays the
+   graph and writes outputs and stats as vmc_batch_run does.  Results equal
+   vmc_batch_run's, bit for bit with VMC_TALLY_FIXED64.  The items of a run
+   must name the plan's problems, photon counts, precision and tally mode,
+   in the plan's order.  Destroy the plan before its problems. */
+typedef struct vmc_batch_plan vmc_batch_plan;
+int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan **out);
+int vmc_batch_plan_run(vmc_batch_plan *plan, vmc_batch_item *items, int32_t n, vmc_stats *stats);
+void vmc_batch_plan_destroy(vmc_batch_plan *plan);
+
 /* One-shot call with the argument list of the MEX gateways
    (cpp/3d/MC3Dmex.cpp:106-110, cpp/2d/MC2Dmex.cpp:105-109): create, run,
    destroy.  HN may be NULL; HN_out (optional) receives the table used.

@@ -174,3 +174,42 @@ def test_many_items_equal_single_runs(vmc):
     finally:
         prob.close()
         probq.close()
+
+
+def test_batch_plan_replays_equal_eager_batches(vmc):
+    """A batch recorded once as a HIP graph (2D and 3D, CW and FD groups, an
+    empty item) and replayed with new seeds, with eager launches between the
+    replays: every replay equals the eager batch of the same seeds bit for
+    bit, a medium changed between replays is used by the next one, and
+    items that do not match the plan are refused."""
+    probs, items = _mix(vmc)
+    kw_common = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+    try:
+        with vmc.BatchPlan(items, **kw_common) as plan:
+            for rep in range(3):
+                seeded = [(prob, dict(kw, seed=kw["seed"] + 100 * rep)) for prob, kw in items]
+                print(f"replay {rep}", flush=True)
+                got, _ = plan.run(seeded)
+                want, _ = vmc.batch_run(seeded, **kw_common)
+                for (_, kw), g, w in zip(seeded, got, want):
+                    for k in OUT:
+                        assert np.array_equal(getattr(g, k), getattr(w, k), equal_nan=True), (rep, kw, k)
+                    assert g.loss == w.loss
+                prob0, kw0 = items[0]
+                prob0.run(5000, seed=rep, precision=vmc.FP32)   # eager work between replays
+            pp, prob = probs["p3_pencil_sphere"]
+            prob.set_medium(mua=np.asarray(pp.mua) * 1.5)
+            print("replay after set_medium", flush=True)
+            got, _ = plan.run()
+            want, _ = vmc.batch_run(items, **kw_common)
+            for (_, kw), g, w in zip(items, got, want):
+                for k in OUT:
+                    assert np.array_equal(getattr(g, k), getattr(w, k), equal_nan=True), ("medium", kw, k)
+            bad = [(prob, dict(kw, nphoton=kw["nphoton"] + 1)) for prob, kw in items]
+            with pytest.raises(vmc.VmcError):
+                plan.run(bad)
+            with pytest.raises(vmc.VmcError):
+                plan.run(items[:-1])
+    finally:
+        for _, prob in probs.values():
+            prob.close()

@@ -105,6 +105,10 @@ def load_library():
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
+        lib.vmc_batch_plan_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        lib.vmc_batch_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Stats)]
+        lib.vmc_batch_plan_destroy.argtypes = [C.c_void_p]
+        lib.vmc_batch_plan_destroy.restype = None
         lib.vmc_problem_autotune.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.POINTER(RunDesc), C.c_int64,
                                              C.POINTER(C.c_int32)]
         lib.vmc_problem_create_variant.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]
@@ -326,12 +330,7 @@ class Problem:
                     replicas=tuned.replicas, lds_slots=tuned.lds_slots, trials=int(n.value))
 
 
-def batch_run(items, **common):
-    """vmc_batch_run: ``items`` is a list of ``(Problem, run_kwargs)``; keyword
-    arguments in ``common`` apply to every item (precision, tally, tuning).
-    Returns one RunResult per item (``steps``, ``kernel_ms`` and the ledger
-    are the batch's) and the batch RunResult-style stats."""
-    lib = load_library()
+def _batch_array(items, common):
     arr = (BatchItem * len(items))()
     outs = []
     for k, (prob, kw) in enumerate(items):
@@ -342,18 +341,74 @@ def batch_run(items, **common):
         out, o = prob._outputs()
         arr[k].out = o
         outs.append(out)
+    return arr, outs
+
+
+def _batch_results(arr, outs, s):
+    return [RunResult(status=int(arr[k].status), loss=int(arr[k].loss), steps=int(s.steps),
+                      kernel_ms=float(s.kernel_ms), total_ms=float(s.total_ms),
+                      ledger=np.array(list(s.ledger)), photons=int(arr[k].run.Nphoton),
+                      grid_blocks=int(s.grid_blocks), block_size=int(s.block_size),
+                      rechecks=int(s.rechecks), **out) for k, out in enumerate(outs)]
+
+
+def batch_run(items, **common):
+    """vmc_batch_run: ``items`` is a list of ``(Problem, run_kwargs)``; keyword
+    arguments in ``common`` apply to every item (precision, tally, tuning).
+    Returns one RunResult per item (``steps``, ``kernel_ms`` and the ledger
+    are the batch's) and the batch RunResult-style stats."""
+    lib = load_library()
+    arr, outs = _batch_array(items, common)
     s = Stats()
     st = lib.vmc_batch_run(arr, len(items), C.byref(s))
     if st:
         raise VmcError(st, lib)
-    res = []
-    for k, out in enumerate(outs):
-        res.append(RunResult(status=int(arr[k].status), loss=int(arr[k].loss), steps=int(s.steps),
-                             kernel_ms=float(s.kernel_ms), total_ms=float(s.total_ms),
-                             ledger=np.array(list(s.ledger)), photons=int(arr[k].run.Nphoton),
-                             grid_blocks=int(s.grid_blocks), block_size=int(s.block_size),
-                             rechecks=int(s.rechecks), **out))
-    return res, s
+    return _batch_results(arr, outs, s), s
+
+
+class BatchPlan:
+    """vmc_batch_plan_*: a batch recorded once as a HIP graph and replayed.
+    ``run(items)`` replays it for ``items`` (default: the ones it was built
+    from), which must name the same problems and photon counts in the same
+    order and may change seeds, stream tags and phases; media changed with
+    ``Problem.set_medium`` are picked up.  Returns what ``batch_run``
+    returns."""
+
+    def __init__(self, items, **common):
+        self.lib = load_library()
+        self._items, self._common = list(items), dict(common)
+        arr, _ = _batch_array(self._items, self._common)
+        h = C.c_void_p()
+        st = self.lib.vmc_batch_plan_create(arr, len(self._items), C.byref(h))
+        if st:
+            raise VmcError(st, self.lib)
+        self._h = h
+
+    def run(self, items=None):
+        items = self._items if items is None else list(items)
+        arr, outs = _batch_array(items, self._common)
+        s = Stats()
+        st = self.lib.vmc_batch_plan_run(self._h, arr, len(items), C.byref(s))
+        if st:
+            raise VmcError(st, self.lib)
+        return _batch_results(arr, outs, s), s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.vmc_batch_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServerDesc(C.Structure):

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 
 namespace vmc {
@@ -87,47 +88,52 @@ void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t 
 
 // Sizes the arena for ne element and nb boundary slots and clears its
 // tallies, ledger, counters and per-item loss counts.
-void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems) {
+void arena_alloc(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems) {
   a.ne = ne;
   a.nb = nb;
   a.W = c.W;
   a.nrep = c.nrep;
   a.nitems = nitems;
-  const size_t ew = (size_t)ne * c.W * c.nrep, bw = (size_t)nb * 4 * c.nrep;
-  a.etally.ensure(ew);
-  a.btally.ensure(bw);
+  a.etally.ensure((size_t)ne * c.W * c.nrep);
+  a.btally.ensure((size_t)nb * 4 * c.nrep);
   a.raw_e.ensure((size_t)ne * c.W);
   a.raw_b.ensure((size_t)nb * 4);
   a.ledger.ensure(8);
   a.counters.ensure(8);
   a.item_loss.ensure(nitems);
   a.items.ensure(nitems);
+}
+
+void arena_fill(Arena &a, const LaunchCfg &c) {
+  const size_t ew = (size_t)a.ne * c.W * c.nrep, bw = (size_t)a.nb * 4 * c.nrep;
   hip_check(launch_fill_u64(a.etally.p, ew, 0ull, a.stream), "fill");
   hip_check(launch_fill_u64(a.btally.p, bw, 0ull, a.stream), "fill");
   hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), 8, 0ull, a.stream), "fill");
   hip_check(launch_fill_u64(a.counters.p, 8, 0ull, a.stream), "fill");
-  hip_check(launch_fill_u64(a.item_loss.p, nitems, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(a.item_loss.p, a.nitems, 0ull, a.stream), "fill");
 }
 
-// One persistent launch over `items` (vstart set here, chunk-aligned and in
-// order).  Returns the kernel time from HIP events on the arena's stream and
-// the grid size.
-float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out) {
+void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems) {
+  arena_alloc(a, c, ne, nb, nitems);
+  arena_fill(a, c);
+}
+
+WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items) {
+  WalkLaunch w;
   uint64_t v = 0;
   for (ItemRec &it : items) {
     it.vstart = v;
     v += (it.count + c.chunk - 1) / c.chunk * c.chunk;
   }
-  if (v == 0) return 0.f;
-  const bool batch = items.size() > 1;
+  if (v == 0) return w;
+  w.batch = items.size() > 1;
   int bpc = 0;
-  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, batch, c.block, c.lds, &bpc), "occupancy");
+  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, w.batch, c.block, c.lds, &bpc), "occupancy");
   if (bpc < 1) throw HipFail{hipErrorInvalidConfiguration, "workgroup does not fit on a CU (block size / LDS tally cache)"};
-  const int grid = bpc * num_cus;
+  w.grid = bpc * num_cus;
   hip_check(hipMemcpyAsync(a.items.p, items.data(), items.size() * sizeof(ItemRec), hipMemcpyHostToDevice, a.stream),
             "upload");
-  hip_check(launch_fill_u64(a.counters.p, 1, 0ull, a.stream), "fill");   // dispenser
-  WalkParams P{};
+  WalkParams &P = w.P;
   P.items = a.items.p;
   P.nitems = (int32_t)items.size();
   P.vtotal = v;
@@ -150,28 +156,41 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
   P.weight0_f = (float)P.weight0;
   P.chance_f = (float)P.chance;
   P.fix_scale = 4294967296.0;
+  return w;
+}
+
+void enqueue_walk(Arena &a, const LaunchCfg &c, const WalkLaunch &w) {
+  if (w.grid == 0) return;
+  hip_check(launch_fill_u64(a.counters.p, 1, 0ull, a.stream), "fill");   // dispenser
+  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, w.batch, w.grid, c.block, c.lds, a.stream, w.P), "walk");
+}
+
+float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out) {
+  const WalkLaunch w = prepare_walk(a, num_cus, c, items);
+  if (w.grid == 0) return 0.f;
   hip_check(hipEventRecord(a.ev0, a.stream), "event");
-  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, batch, grid, c.block, c.lds, a.stream, P), "walk");
+  enqueue_walk(a, c, w);
   hip_check(hipEventRecord(a.ev1, a.stream), "event");
   hip_check(hipEventSynchronize(a.ev1), "sync");
   float ms = 0.f;
   hip_check(hipEventElapsedTime(&ms, a.ev0, a.ev1), "event");
-  if (grid_out) *grid_out = grid;
+  if (grid_out) *grid_out = w.grid;
   return ms;
 }
 
-// Sums the replicas and copies raw element words [Ne_pad*W], boundary
-// words [nb*4], the ledger, the counters and the per-item loss counts to
-// the host.
-void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+void arena_reduce(Arena &a, const LaunchCfg &c) {
   const double inv = c.fixed ? 1.0 / 4294967296.0 : 1.0;
+  const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
+  hip_check(launch_reduce(c.fixed, a.etally.p, a.raw_e.p, ew, c.nrep, ew, inv, a.stream), "reduce");
+  hip_check(launch_reduce(c.fixed, a.btally.p, a.raw_b.p, bw, c.nrep, bw, inv, a.stream), "reduce");
+}
+
+void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
+                    double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
   const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
   raw_e.resize(ew);
   raw_b.resize(bw);
   loss.resize(a.nitems);
-  hip_check(launch_reduce(c.fixed, a.etally.p, a.raw_e.p, ew, c.nrep, ew, inv, a.stream), "reduce");
-  hip_check(launch_reduce(c.fixed, a.btally.p, a.raw_b.p, bw, c.nrep, bw, inv, a.stream), "reduce");
   hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, ew * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, bw * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(led, a.ledger.p, 5 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
@@ -179,6 +198,12 @@ void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std
   if (a.nitems)
     hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, a.nitems * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipStreamSynchronize(a.stream), "sync");
+}
+
+void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
+                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+  arena_reduce(a, c);
+  arena_copy_out(a, c, raw_e, raw_b, led, cnt, loss);
 }
 
 // Item record of a problem run: its device arrays, stream key and phase.
@@ -242,8 +267,109 @@ Arena g_batch_arena;   // tallies of vmc_batch_run (one batch at a time)
 std::mutex g_batch_mutex;
 } // namespace
 
+// One (dimension, CW/FD) group of a batch: its launch settings, the batch
+// positions of its items and their element / boundary offsets in the arena.
+struct BatchGroup {
+  LaunchCfg c;
+  std::vector<int32_t> idx;
+  std::vector<int64_t> tb, bb;
+  int64_t ne = 0, nb = 0;
+};
+
+// Checks a batch (problems on one device, one precision and tally mode),
+// splits it into groups in launch order with the settings of items[0].run,
+// and uploads the problems' element records; a status.
+int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &groups) {
+  if (!items || n <= 0 || !items[0].prob) return VMC_ERR_INVALID_ARGUMENT;
+  const vmc_problem *m0 = items[0].prob;
+  for (int32_t i = 0; i < n; ++i) {
+    const vmc_batch_item &b = items[i];
+    if (!b.prob || b.run.Nphoton < 0 || b.prob->device != m0->device) return VMC_ERR_INVALID_ARGUMENT;
+    if (b.run.precision != items[0].run.precision || b.run.tally != items[0].run.tally)
+      return VMC_ERR_INVALID_ARGUMENT;
+  }
+  try {
+    hip_check(hipSetDevice(m0->device), "hipSetDevice");
+    for (int dim = 2; dim <= 3; ++dim) {
+      for (int fdi = 0; fdi < 2; ++fdi) {
+        BatchGroup g;
+        for (int32_t i = 0; i < n; ++i)
+          if (items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1))) g.idx.push_back(i);
+        if (g.idx.empty()) continue;
+        const int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &g.c);
+        if (st != VMC_OK) return st;
+        for (int32_t i : g.idx) {
+          vmc_problem &m = *items[i].prob;
+          upload_params(m, g.c.f64);
+          g.tb.push_back(g.ne);
+          g.bb.push_back(g.nb);
+          g.ne += sector_round(m.Ne);
+          g.nb += m.Nb;
+        }
+        if (g.ne > ((int64_t)1 << 30) / g.c.W || g.nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
+        choose_replicas(g.c, &items[0].run, g.ne, g.nb);
+        groups.push_back(std::move(g));
+      }
+    }
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  return VMC_OK;
+}
+
+// Normalised outputs and loss counts of a group's items from its raw words;
+// adds the group's counters and ledger to *tot.
+void finish_group(vmc_batch_item *items, const BatchGroup &g, const std::vector<double> &raw_e,
+                  const std::vector<double> &raw_b, const double led[5], const unsigned long long cnt[4],
+                  const std::vector<unsigned long long> &loss, vmc_stats *tot) {
+  for (size_t k = 0; k < g.idx.size(); ++k) {
+    vmc_batch_item &b = items[g.idx[k]];
+    finish_outputs(*b.prob, b.run.f, b.run.Nphoton, g.c.W, raw_e.data() + g.tb[k] * g.c.W,
+                   raw_b.data() + g.bb[k] * 4, &b.out);
+    b.loss = (int64_t)loss[k];
+    tot->photons += b.run.Nphoton;
+  }
+  tot->steps += (int64_t)cnt[1];
+  tot->loss += (int64_t)cnt[2];
+  tot->rechecks += (int64_t)cnt[3];
+  for (int q = 0; q < 5; ++q) tot->ledger[q] += led[q];
+  tot->block_size = g.c.block;
+}
+
 
 } // namespace vmc
+
+// A batch recorded as a HIP graph (vmc_batch_plan_*): per group an arena on
+// the plan's stream, the items' records in device order and the launch.
+struct vmc_batch_plan {
+  struct Group {
+    vmc::BatchGroup g;
+    vmc::Arena arena;
+    std::vector<vmc::ItemRec> recs;
+    vmc::WalkLaunch w;
+  };
+  int device = 0, num_cus = 0;
+  int32_t precision = 0, tally = 0;
+  std::vector<vmc_problem *> probs;
+  std::vector<int64_t> counts;
+  std::vector<bool> fd;
+  std::vector<std::unique_ptr<Group>> groups;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~vmc_batch_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &pg : groups) {
+      pg->arena.stream = nullptr;   // the plan's, destroyed below
+      pg->arena.close();
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
 
 using namespace vmc;
 
@@ -504,74 +630,38 @@ int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *
 }
 
 int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
-  if (!items || n <= 0) return VMC_ERR_INVALID_ARGUMENT;
   const auto t_start = std::chrono::steady_clock::now();
-  vmc_problem *m0 = items[0].prob;
-  if (!m0) return VMC_ERR_INVALID_ARGUMENT;
-  // one launch per (dimension, CW/FD) group; precision, tally mode and
-  // tuning come from the first item and must agree across the batch
+  std::vector<BatchGroup> groups;
+  {
+    const int st = plan_batch(items, n, groups);
+    if (st != VMC_OK) return st;
+  }
   for (int32_t i = 0; i < n; ++i) {
-    const vmc_batch_item &b = items[i];
-    if (!b.prob || b.run.Nphoton < 0 || b.prob->device != m0->device) return VMC_ERR_INVALID_ARGUMENT;
-    if (b.run.precision != items[0].run.precision || b.run.tally != items[0].run.tally)
-      return VMC_ERR_INVALID_ARGUMENT;
     items[i].loss = 0;
     items[i].status = VMC_OK;
   }
+  vmc_problem *m0 = items[0].prob;
   vmc_stats tot{};
   float kernel_ms = 0.f;
   try {
-    hip_check(hipSetDevice(m0->device), "hipSetDevice");
     std::lock_guard<std::mutex> lock(g_batch_mutex);
     if (!g_batch_arena.stream) g_batch_arena.open();
-    for (int dim = 2; dim <= 3; ++dim) {
-      for (int fdi = 0; fdi < 2; ++fdi) {
-        std::vector<int32_t> group;
-        for (int32_t i = 0; i < n; ++i)
-          if (items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1))) group.push_back(i);
-        if (group.empty()) continue;
-        LaunchCfg c;
-        int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &c);
-        if (st != VMC_OK) return st;
-        int64_t ne = 0, nb = 0;
-        std::vector<int64_t> tb(group.size()), bb(group.size());
-        for (size_t g = 0; g < group.size(); ++g) {
-          vmc_problem &m = *items[group[g]].prob;
-          upload_params(m, c.f64);
-          tb[g] = ne;
-          bb[g] = nb;
-          ne += sector_round(m.Ne);
-          nb += m.Nb;
-        }
-        if (ne > ((int64_t)1 << 30) / c.W || nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
-        choose_replicas(c, &items[0].run, ne, nb);
-        arena_reset(g_batch_arena, c, ne, nb, group.size());
-        std::vector<ItemRec> recs;
-        for (size_t g = 0; g < group.size(); ++g) {
-          const vmc_batch_item &b = items[group[g]];
-          recs.push_back(make_item(*b.prob, &b.run, c.f64, (uint64_t)b.run.Nphoton, 0, tb[g], bb[g]));
-        }
-        int grid = 0;
-        kernel_ms += launch_items(g_batch_arena, m0->num_cus, c, recs, &grid);
-        std::vector<double> raw_e, raw_b;
-        std::vector<unsigned long long> loss;
-        double led[5];
-        unsigned long long cnt[4];
-        arena_collect(g_batch_arena, c, raw_e, raw_b, led, cnt, loss);
-        for (size_t g = 0; g < group.size(); ++g) {
-          vmc_batch_item &b = items[group[g]];
-          finish_outputs(*b.prob, b.run.f, b.run.Nphoton, c.W, raw_e.data() + tb[g] * c.W, raw_b.data() + bb[g] * 4,
-                         &b.out);
-          b.loss = (int64_t)loss[g];
-          tot.photons += b.run.Nphoton;
-        }
-        tot.steps += (int64_t)cnt[1];
-        tot.loss += (int64_t)cnt[2];
-        tot.rechecks += (int64_t)cnt[3];
-        for (int q = 0; q < 5; ++q) tot.ledger[q] += led[q];
-        tot.grid_blocks = grid;
-        tot.block_size = c.block;
+    for (const BatchGroup &g : groups) {
+      arena_reset(g_batch_arena, g.c, g.ne, g.nb, g.idx.size());
+      std::vector<ItemRec> recs;
+      for (size_t k = 0; k < g.idx.size(); ++k) {
+        const vmc_batch_item &b = items[g.idx[k]];
+        recs.push_back(make_item(*b.prob, &b.run, g.c.f64, (uint64_t)b.run.Nphoton, 0, g.tb[k], g.bb[k]));
       }
+      int grid = 0;
+      kernel_ms += launch_items(g_batch_arena, m0->num_cus, g.c, recs, &grid);
+      std::vector<double> raw_e, raw_b;
+      std::vector<unsigned long long> loss;
+      double led[5];
+      unsigned long long cnt[4];
+      arena_collect(g_batch_arena, g.c, raw_e, raw_b, led, cnt, loss);
+      finish_group(items, g, raw_e, raw_b, led, cnt, loss, &tot);
+      tot.grid_blocks = grid;
     }
   } catch (HipFail f) {
     return fail_hip(f);
@@ -584,6 +674,122 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
   }
   return VMC_OK;
 }
+
+int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan **out) {
+  if (!out) return VMC_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::vector<BatchGroup> groups;
+  {
+    const int st = plan_batch(items, n, groups);
+    if (st != VMC_OK) return st;
+  }
+  std::unique_ptr<vmc_batch_plan> pl(new vmc_batch_plan);
+  pl->device = items[0].prob->device;
+  pl->num_cus = items[0].prob->num_cus;
+  pl->precision = items[0].run.precision;
+  pl->tally = items[0].run.tally;
+  for (int32_t i = 0; i < n; ++i) {
+    pl->probs.push_back(items[i].prob);
+    pl->counts.push_back(items[i].run.Nphoton);
+    pl->fd.push_back(2.0 * M_PI * items[i].run.f > 0.0);
+  }
+  try {
+    hip_check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&pl->ev0), "hipEventCreate");
+    hip_check(hipEventCreate(&pl->ev1), "hipEventCreate");
+    for (BatchGroup &g : groups) {
+      std::unique_ptr<vmc_batch_plan::Group> pg(new vmc_batch_plan::Group);
+      pg->g = g;
+      pg->arena.stream = pl->stream;   // the plan's stream, not the arena's own
+      arena_alloc(pg->arena, g.c, g.ne, g.nb, g.idx.size());
+      for (size_t k = 0; k < g.idx.size(); ++k) {
+        const vmc_batch_item &b = items[g.idx[k]];
+        pg->recs.push_back(make_item(*b.prob, &b.run, g.c.f64, (uint64_t)b.run.Nphoton, 0, g.tb[k], g.bb[k]));
+      }
+      pg->w = prepare_walk(pg->arena, pl->num_cus, g.c, pg->recs);
+      pl->groups.push_back(std::move(pg));
+    }
+    hip_check(hipStreamSynchronize(pl->stream), "sync");
+    // the device work of a run, recorded once: fill kernels (no memset
+    // nodes), the walk launches and the replica sums
+    hip_check(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+      for (auto &pg : pl->groups) {
+        arena_fill(pg->arena, pg->g.c);
+        enqueue_walk(pg->arena, pg->g.c, pg->w);
+        arena_reduce(pg->arena, pg->g.c);
+      }
+    } catch (HipFail) {
+      hipGraph_t partial = nullptr;
+      (void)hipStreamEndCapture(pl->stream, &partial);
+      if (partial) (void)hipGraphDestroy(partial);
+      throw;
+    }
+    hip_check(hipStreamEndCapture(pl->stream, &pl->graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  *out = pl.release();
+  return VMC_OK;
+}
+
+int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc_stats *stats) {
+  if (!pl || !items || n != (int32_t)pl->probs.size()) return VMC_ERR_INVALID_ARGUMENT;
+  for (int32_t i = 0; i < n; ++i) {
+    const vmc_run_desc &r = items[i].run;
+    if (items[i].prob != pl->probs[i] || r.Nphoton != pl->counts[i] || r.precision != pl->precision ||
+        r.tally != pl->tally || (2.0 * M_PI * r.f > 0.0) != pl->fd[i])
+      return VMC_ERR_INVALID_ARGUMENT;
+    items[i].loss = 0;
+    items[i].status = VMC_OK;
+  }
+  const auto t_start = std::chrono::steady_clock::now();
+  vmc_stats tot{};
+  float kernel_ms = 0.f;
+  try {
+    hip_check(hipSetDevice(pl->device), "hipSetDevice");
+    // current records: seeds, tags, phases and the problems' current media
+    for (auto &pg : pl->groups) {
+      const BatchGroup &g = pg->g;
+      for (size_t k = 0; k < g.idx.size(); ++k) {
+        const vmc_batch_item &b = items[g.idx[k]];
+        upload_params(*b.prob, g.c.f64);
+        ItemRec it = make_item(*b.prob, &b.run, g.c.f64, (uint64_t)b.run.Nphoton, 0, g.tb[k], g.bb[k]);
+        it.vstart = pg->recs[k].vstart;
+        pg->recs[k] = it;
+      }
+      hip_check(hipMemcpyAsync(pg->arena.items.p, pg->recs.data(), pg->recs.size() * sizeof(ItemRec),
+                               hipMemcpyHostToDevice, pl->stream),
+                "upload");
+    }
+    hip_check(hipEventRecord(pl->ev0, pl->stream), "event");
+    hip_check(hipGraphLaunch(pl->exec, pl->stream), "hipGraphLaunch");
+    hip_check(hipEventRecord(pl->ev1, pl->stream), "event");
+    hip_check(hipEventSynchronize(pl->ev1), "sync");
+    hip_check(hipEventElapsedTime(&kernel_ms, pl->ev0, pl->ev1), "event");
+    for (auto &pg : pl->groups) {
+      std::vector<double> raw_e, raw_b;
+      std::vector<unsigned long long> loss;
+      double led[5];
+      unsigned long long cnt[4];
+      arena_copy_out(pg->arena, pg->g.c, raw_e, raw_b, led, cnt, loss);
+      finish_group(items, pg->g, raw_e, raw_b, led, cnt, loss, &tot);
+      tot.grid_blocks = pg->w.grid;
+    }
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  if (stats) {
+    *stats = tot;
+    stats->kernel_ms = kernel_ms;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  return VMC_OK;
+}
+
+void vmc_batch_plan_destroy(vmc_batch_plan *pl) { delete pl; }
 
 void vmc_problem_destroy(vmc_problem *m) {
   if (!m) return;

@@ -215,10 +215,36 @@ struct LaunchCfg {
 
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c);
 void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb);
-void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems);
+// The arena's buffers for ne element and nb boundary slots and nitems
+// items (no device work; nothing is allocated under stream capture).
+void arena_alloc(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems);
+// Enqueues the clearing of its tallies, ledger, counters and loss counts
+// (fill kernels, so a captured graph records no memset).
+void arena_fill(Arena &a, const LaunchCfg &c);
+void arena_reset(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems);   // alloc + fill
+
+// A persistent walk launch over a list of items: prepare_walk sets the
+// items' virtual photon ranges (chunk-aligned, in order), uploads the item
+// records and fills in the kernel parameters (grid 0: no photons);
+// enqueue_walk enqueues the dispenser reset and the kernel.
+struct WalkLaunch {
+  WalkParams P{};
+  int grid = 0;
+  bool batch = false;
+};
+WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items);
+void enqueue_walk(Arena &a, const LaunchCfg &c, const WalkLaunch &w);
+// prepare + enqueue + wait; returns the kernel time (HIP events on the arena's stream)
 float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out);
+
+// Enqueues the replica sums into raw element words [ne*W] and boundary
+// words [nb*4]; arena_copy_out copies them, the ledger, the counters and the
+// per-item loss counts to the host and waits.
+void arena_reduce(Arena &a, const LaunchCfg &c);
+void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
+                    double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss);
 void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss);
+                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss);   // reduce + copy out
 ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint64_t count, uint64_t id0,
                   int64_t tbase, int64_t bbase);
 void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
