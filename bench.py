@@ -136,10 +136,12 @@ def cpu_baseline(p, target_s: float, what: str = "the same problem"):
 
 
 def run_batch(a):
-    """256 forward solves on the 384k-tet box in one persistent launch
-    (FP32 transport, separate FP64 tallies per instance), against the same
-    items run one vmc_problem_run at a time, and replayed from a batch plan
-    (the launch recorded once as a HIP graph)."""
+    """256 forward solves on the 384k-tet box (64 pencil positions x 4
+    optical-parameter sets, FP32 transport, separate FP64 tallies per
+    instance): launch knobs picked by vmc_batch_autotune on the batch
+    itself, the batch recorded once as a HIP graph (vmc_batch_plan) and
+    replayed -- the timed figure -- next to the same batch launched eagerly
+    and the same items run one vmc_problem_run at a time."""
     import valomc_amd as vmc
     from valomc_amd import meshgen
     base_p, sources, media, photons = meshgen.batched_box_instances(seed=1234, photon_scale=a.photon_scale)
@@ -154,12 +156,28 @@ def run_batch(a):
               tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
     small = [(pr, dict(nphoton=max(1, k["nphoton"] // 100), seed=k["seed"])) for pr, k in items]
     vmc.batch_run(small, **kw)   # warm-up
+    t0 = time.perf_counter()
+    tuned = vmc.batch_autotune(items, **kw)
+    tuned["tuning_s"] = round(time.perf_counter() - t0, 3)
+    kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots")})
+    # the batch as a HIP graph: recorded once, replayed (a warm-up replay
+    # with other seeds, then the timed replays)
+    t0 = time.perf_counter()
+    plan = vmc.BatchPlan(items, **kw)
+    plan_s = time.perf_counter() - t0
+    plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
     walls = []
     for _ in range(a.steps):
         t0 = time.perf_counter()
-        res, st = vmc.batch_run(items, **kw)
+        res, st = plan.run()
         walls.append(time.perf_counter() - t0)
     wall = sum(walls) / len(walls)
+    plan.close()
+    # the same batch launched eagerly
+    t0 = time.perf_counter()
+    eres, est = vmc.batch_run(items, **kw)
+    eager_wall = time.perf_counter() - t0
+    graph_same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, eres))
     # the same items one solve at a time
     t0 = time.perf_counter()
     seq_kernel = 0.0
@@ -170,17 +188,6 @@ def run_batch(a):
         seq.append(r)
     seq_wall = time.perf_counter() - t0
     same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, seq))
-    # the same batch recorded once as a HIP graph and replayed (new seeds on
-    # the first replay, then the timed seeds)
-    t0 = time.perf_counter()
-    plan = vmc.BatchPlan(items, **kw)
-    plan_s = time.perf_counter() - t0
-    plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
-    t0 = time.perf_counter()
-    gres, gst = plan.run()
-    graph_wall = time.perf_counter() - t0
-    plan.close()
-    graph_same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(gres, res))
     # accuracy: FP32 batch items against FP64 runs of the same streams (first 4)
     rel = []
     for (pr, k), r in list(zip(items, res))[:4]:
@@ -191,18 +198,19 @@ def run_batch(a):
         "steps": a.steps, "warmup": 1, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "256 instances on the 384k-tet box (64 pencil positions x 4 optical-parameter "
-                               "sets), one persistent launch, FP32 transport, separate FP64 tallies",
+                               "sets), one persistent launch replayed from a HIP graph, autotuned, FP32 "
+                               "transport, separate FP64 tallies",
                    "instances": len(items), "photons_per_step": total, "photon_scale": a.photon_scale,
                    "parallelism": "single GPU"},
+        "autotune": tuned,
+        "graph_plan_create_s": round(plan_s, 3),
         "kernel_ms_per_step": round(st.kernel_ms, 3),
         "element_steps_per_s": round(st.steps / (st.kernel_ms / 1e3), 1),
+        "eager_batch_photons_per_s": round(total / eager_wall, 1),
+        "graph_vs_eager_rel_l2_max": graph_same,
         "sequential_photons_per_s": round(total / seq_wall, 1),
         "sequential_kernel_ms": round(seq_kernel, 3),
         "batch_vs_sequential_rel_l2_max": same,
-        "graph_replay_photons_per_s": round(total / graph_wall, 1),
-        "graph_replay_kernel_ms": round(gst.kernel_ms, 3),
-        "graph_plan_create_s": round(plan_s, 3),
-        "graph_vs_eager_rel_l2_max": graph_same,
         "fp32_vs_fp64_rel_l2_first4": rel,
     }
     print(json.dumps(line), flush=True)

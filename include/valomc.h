@@ -200,6 +200,14 @@ typedef struct vmc_batch_item {
    sums the whole batch. */
 int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats);
 
+/* vmc_problem_autotune for a batch: times vmc_batch_run on the same items
+   with trial_fraction of their photons (0 = 2 %) over the same knobs and
+   writes items[0].run with the fastest values into *tuned; apply its
+   block_size, refill_min, chunk, replicas and lds_slots to every item.  No
+   counterpart in the reference. */
+int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_fraction, vmc_run_desc *tuned,
+                       int32_t *trials);
+
 /* A batch recorded once as a HIP graph and replayed; no counterpart in the
    reference.  For loops that run the same batch again and again with new
    seeds or media (an inverse solver's iterations).  Creation checks the

@@ -213,3 +213,23 @@ def test_batch_plan_replays_equal_eager_batches(vmc):
     finally:
         for _, prob in probs.values():
             prob.close()
+
+
+def test_batch_autotune_keeps_results(vmc):
+    """vmc_batch_autotune picks valid knobs for a mixed batch, and the batch
+    run with them equals the batch run with the defaults (fixed point)."""
+    probs, items = _mix(vmc)
+    common = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+    try:
+        t = vmc.batch_autotune(items, trial_fraction=0.5, **common)
+        assert t.pop("trials") >= 10
+        assert t["block_size"] in (1024, 512, 256) and t["chunk"] in (128, 64, 256, 512)
+        assert t["refill_min"] in (6, 2, 16, 32) and t["replicas"] in (0, 8, 1) and t["lds_slots"] in (0, -1)
+        base, _ = vmc.batch_run(items, **common)
+        tuned, _ = vmc.batch_run(items, **common, **t)
+        for (_, kw), r0, r1 in zip(items, base, tuned):
+            for k in OUT:
+                assert np.array_equal(getattr(r0, k), getattr(r1, k), equal_nan=True), (t, kw, k)
+    finally:
+        for _, prob in probs.values():
+            prob.close()

@@ -105,6 +105,8 @@ def load_library():
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
+        lib.vmc_batch_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.POINTER(RunDesc),
+                                           C.POINTER(C.c_int32)]
         lib.vmc_batch_plan_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
         lib.vmc_batch_plan_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(Stats)]
         lib.vmc_batch_plan_destroy.argtypes = [C.c_void_p]
@@ -364,6 +366,22 @@ def batch_run(items, **common):
     if st:
         raise VmcError(st, lib)
     return _batch_results(arr, outs, s), s
+
+
+def batch_autotune(items, trial_fraction: float = 0.0, **common) -> dict:
+    """vmc_batch_autotune: launch knobs for ``batch_run(items, **common)``
+    timed on the same batch with ``trial_fraction`` of its photons (0 = 2 %);
+    returns keyword arguments (block_size, refill_min, chunk, replicas,
+    lds_slots) to add to ``common``, plus ``trials``."""
+    lib = load_library()
+    arr, _ = _batch_array(items, common)
+    tuned = RunDesc()
+    n = C.c_int32(0)
+    st = lib.vmc_batch_autotune(arr, len(items), C.c_double(float(trial_fraction)), C.byref(tuned), C.byref(n))
+    if st:
+        raise VmcError(st, lib)
+    return dict(block_size=tuned.block_size, refill_min=tuned.refill_min, chunk=tuned.chunk,
+                replicas=tuned.replicas, lds_slots=tuned.lds_slots, trials=int(n.value))
 
 
 class BatchPlan:

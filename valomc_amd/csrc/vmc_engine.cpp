@@ -235,6 +235,7 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
 // (MC3D.hpp:1793-1818).
 void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
                     vmc_output *out) {
+  if (!out || (!out->ER && !out->EI && !out->EBR && !out->EBI && !out->DEBR && !out->DEBI)) return;
   const double omega = 2.0 * M_PI * f;
   const bool fd = omega > 0.0;
   std::vector<double> ER(m.Ne), EI(m.Ne, 0.0), EBR(m.Nb), EBI(m.Nb), DEBR(m.Nb), DEBI(m.Nb), kn(m.Ne);
@@ -266,6 +267,54 @@ namespace {
 Arena g_batch_arena;   // tallies of vmc_batch_run (one batch at a time)
 std::mutex g_batch_mutex;
 } // namespace
+
+// Coordinate descent over the launch knobs from `base`: each knob's values
+// are timed with the others fixed (`time_of` returns kernel milliseconds,
+// +inf when the device refuses), and a knob moves only for a gain above 2 %.
+// The first timing of the base configuration only warms up (arena
+// allocation, code loading).
+template <class TimeOf>
+int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trials, TimeOf time_of) {
+  int count = 0;
+  auto timed = [&](const vmc_run_desc &c) {
+    ++count;
+    return time_of(c);
+  };
+  vmc_run_desc best = base;
+  const bool f64 = base.precision == VMC_FP64;
+  // defaults spelled out, so a knob's default value is not timed twice
+  if (best.block_size == 0) best.block_size = f64 ? 256 : 1024;
+  if (best.refill_min == 0) best.refill_min = 6;
+  if (best.chunk == 0) best.chunk = (int32_t)kChunk;
+  if (!std::isfinite(timed(best))) return VMC_ERR_INVALID_ARGUMENT;
+  double tb = timed(best);
+  struct Knob {
+    int32_t vmc_run_desc::*field;
+    std::vector<int32_t> values;
+  };
+  const std::vector<Knob> knobs = {
+      {&vmc_run_desc::block_size, f64 ? std::vector<int32_t>{256, 128, 64} : std::vector<int32_t>{1024, 512, 256}},
+      {&vmc_run_desc::refill_min, {6, 2, 16, 32}},
+      {&vmc_run_desc::chunk, {128, 64, 256, 512}},
+      {&vmc_run_desc::replicas, {0, 8, 1}},
+      {&vmc_run_desc::lds_slots, {0, -1}},
+  };
+  for (const Knob &k : knobs) {
+    for (int32_t v : k.values) {
+      if (best.*(k.field) == v) continue;
+      vmc_run_desc cand = best;
+      cand.*(k.field) = v;
+      const double t = timed(cand);
+      if (t < 0.98 * tb) {
+        best = cand;
+        tb = t;
+      }
+    }
+  }
+  *tuned = best;
+  if (trials) *trials = count;
+  return VMC_OK;
+}
 
 // One (dimension, CW/FD) group of a batch: its launch settings, the batch
 // positions of its items and their element / boundary offsets in the arena.
@@ -574,59 +623,52 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
 int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *tuned, int64_t trial_photons,
                          int32_t *trials) {
   if (!m || !run || !tuned || run->Nphoton < 0 || trial_photons < 0) return VMC_ERR_INVALID_ARGUMENT;
-  const int64_t n = trial_photons > 0 ? trial_photons
-                                      : std::min<int64_t>(10000000, std::max<int64_t>(100000, run->Nphoton / 50));
-  vmc_output none{};
-  int count = 0;
-  // kernel time of a trial; +inf for a configuration the device refuses
-  auto time_of = [&](vmc_run_desc cand) {
-    cand.Nphoton = n;
-    vmc_stats st{};
-    ++count;
-    const int s = vmc_problem_run(m, &cand, &none, &st, nullptr, nullptr);
-    return s == VMC_OK ? st.kernel_ms : std::numeric_limits<double>::infinity();
-  };
-  vmc_run_desc best = *run;
   {
     LaunchCfg c;
     const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, 2.0 * M_PI * run->f > 0.0, run, &c);
     if (st != VMC_OK) return st;
   }
-  // first run of a configuration allocates the arena and loads the code
-  if (!std::isfinite(time_of(best))) return VMC_ERR_INVALID_ARGUMENT;
-  const bool f64 = run->precision == VMC_FP64;
-  // defaults spelled out, so a knob's default value is not timed twice
-  if (best.block_size == 0) best.block_size = f64 ? 256 : 1024;
-  if (best.refill_min == 0) best.refill_min = 6;
-  if (best.chunk == 0) best.chunk = (int32_t)kChunk;
-  double tb = time_of(best);
-  struct Knob {
-    int32_t vmc_run_desc::*field;
-    std::vector<int32_t> values;
-  };
-  const std::vector<Knob> knobs = {
-      {&vmc_run_desc::block_size, f64 ? std::vector<int32_t>{256, 128, 64} : std::vector<int32_t>{1024, 512, 256}},
-      {&vmc_run_desc::refill_min, {6, 2, 16, 32}},
-      {&vmc_run_desc::chunk, {128, 64, 256, 512}},
-      {&vmc_run_desc::replicas, {0, 8, 1}},
-      {&vmc_run_desc::lds_slots, {0, -1}},
-  };
-  for (const Knob &k : knobs) {
-    for (int32_t v : k.values) {
-      if (best.*(k.field) == v) continue;
-      vmc_run_desc cand = best;
-      cand.*(k.field) = v;
-      const double t = time_of(cand);
-      if (t < 0.98 * tb) {
-        best = cand;
-        tb = t;
-      }
-    }
-  }
-  *tuned = best;
+  const int64_t n = trial_photons > 0 ? trial_photons
+                                      : std::min<int64_t>(10000000, std::max<int64_t>(100000, run->Nphoton / 50));
+  vmc_output none{};
+  const int st = autotune_knobs(*run, tuned, trials, [&](vmc_run_desc cand) {
+    cand.Nphoton = n;
+    vmc_stats s{};
+    return vmc_problem_run(m, &cand, &none, &s, nullptr, nullptr) == VMC_OK ? s.kernel_ms
+                                                                           : std::numeric_limits<double>::infinity();
+  });
   tuned->Nphoton = run->Nphoton;
-  if (trials) *trials = count;
-  return VMC_OK;
+  return st;
+}
+
+int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_fraction, vmc_run_desc *tuned,
+                       int32_t *trials) {
+  if (!items || n <= 0 || !tuned || !(trial_fraction >= 0.0 && trial_fraction <= 1.0)) return VMC_ERR_INVALID_ARGUMENT;
+  const double frac = trial_fraction > 0.0 ? trial_fraction : 0.02;
+  // trial batches: the same items with a fraction of their photons and no
+  // outputs
+  std::vector<vmc_batch_item> trial(items, items + n);
+  for (vmc_batch_item &b : trial) {
+    if (!b.prob) return VMC_ERR_INVALID_ARGUMENT;
+    if (b.run.Nphoton > 0) b.run.Nphoton = std::max<int64_t>(1, (int64_t)(frac * (double)b.run.Nphoton));
+    b.out = vmc_output{};
+  }
+  return autotune_knobs(items[0].run, tuned, trials, [&](vmc_run_desc cand) {
+    for (vmc_batch_item &b : trial) {
+      const int64_t keep = b.run.Nphoton;
+      const uint64_t seed = b.run.seed;
+      const uint32_t tag = b.run.stream_tag;
+      const double f = b.run.f, ph = b.run.phase0;
+      b.run = cand;
+      b.run.Nphoton = keep;
+      b.run.seed = seed;
+      b.run.stream_tag = tag;
+      b.run.f = f;
+      b.run.phase0 = ph;
+    }
+    vmc_stats s{};
+    return vmc_batch_run(trial.data(), n, &s) == VMC_OK ? s.kernel_ms : std::numeric_limits<double>::infinity();
+  });
 }
 
 int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
