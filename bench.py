@@ -159,7 +159,7 @@ def run_batch(a):
     t0 = time.perf_counter()
     tuned = vmc.batch_autotune(items, **kw)
     tuned["tuning_s"] = round(time.perf_counter() - t0, 3)
-    kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots")})
+    kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots", "cache_epoch")})
     # the batch as a HIP graph: recorded once, replayed (a warm-up replay
     # with other seeds, then the timed replays)
     t0 = time.perf_counter()
@@ -318,7 +318,7 @@ def main():
         t = time.perf_counter()
         tuned = prob.autotune(nphot, precision=vmc.FP32, **kw)
         tuned["tuning_s"] = round(time.perf_counter() - t, 3)
-        kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots")})
+        kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots", "cache_epoch")})
     for i in range(a.warmup):
         prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)
     runs = []

@@ -90,7 +90,11 @@ typedef struct vmc_run_desc {
                             1 = each lane adds its own words (for testing)   */
   int32_t chunk;         /* photon ids a wave takes from the dispenser at a
                             time: 0 = 128, else a power of 2 in [64, 4096]   */
-  int32_t reserved[2];
+  int32_t cache_epoch;   /* loop passes after which a workgroup hands its LDS
+                            sector cache to the replica and lets sectors claim
+                            slots again: 0 = default (1024 for batches of
+                            several items, never for one), -1 = never       */
+  int32_t reserved[1];
 } vmc_run_desc;
 
 typedef struct vmc_output {
@@ -133,7 +137,8 @@ int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
    Times short runs of *run on prob -- trial_photons photons each, 0 = 2 %
    of run->Nphoton within [1e5, 1e7] -- over the launch shape (block_size,
    refill_min, chunk: photons per wave fetch) and the tally strategy
-   (lds_slots cache on/off, replicas), one knob at a time from the defaults,
+   (lds_slots cache on/off, cache_epoch, replicas), one knob at a time from
+   the defaults,
    and writes *run with the fastest values into *tuned.  A knob moves only
    for a gain above 2 %.  Results do not depend on these knobs (bit for bit
    with VMC_TALLY_FIXED64).  trials (optional) receives the number of timed
@@ -203,7 +208,8 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats);
 /* vmc_problem_autotune for a batch: times vmc_batch_run on the same items
    with trial_fraction of their photons (0 = 2 %) over the same knobs and
    writes items[0].run with the fastest values into *tuned; apply its
-   block_size, refill_min, chunk, replicas and lds_slots to every item.  No
+   block_size, refill_min, chunk, replicas, lds_slots and cache_epoch to
+   every item.  No
    counterpart in the reference. */
 int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_fraction, vmc_run_desc *tuned,
                        int32_t *trials);

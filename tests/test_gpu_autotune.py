@@ -40,6 +40,7 @@ def test_autotuned_settings_are_valid_and_keep_results(vmc, name, precision):
         assert t["chunk"] in (128, 64, 256, 512)
         assert t["replicas"] in (0, 8, 1)
         assert t["lds_slots"] in (0, -1)
+        assert t["cache_epoch"] in (0, -1, 256, 4096)
         trials = t.pop("trials")
         assert trials > 0
         base = prob.run(40000, **kw)

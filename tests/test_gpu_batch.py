@@ -225,6 +225,7 @@ def test_batch_autotune_keeps_results(vmc):
         assert t.pop("trials") >= 10
         assert t["block_size"] in (1024, 512, 256) and t["chunk"] in (128, 64, 256, 512)
         assert t["refill_min"] in (6, 2, 16, 32) and t["replicas"] in (0, 8, 1) and t["lds_slots"] in (0, -1)
+        assert t["cache_epoch"] in (0, -1, 256, 4096)
         base, _ = vmc.batch_run(items, **common)
         tuned, _ = vmc.batch_run(items, **common, **t)
         for (_, kw), r0, r1 in zip(items, base, tuned):

@@ -118,8 +118,9 @@ def test_fixed_point_tallies_are_bit_reproducible(vmc):
 @pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_iso_gauss_fd",
                                   "p2_mirror_hetero"])
 def test_tally_paths_agree_bit_for_bit(vmc, name):
-    """Fixed-point adds are exact and order-independent, so the LDS cache,
-    the coalesced flush and per-lane atomics must give the same bits, for
+    """Fixed-point adds are exact and order-independent, so the LDS cache
+    (kept for the whole launch or handed back every few loop passes), the
+    coalesced flush and per-lane atomics must give the same bits, for
     element words (CW and FD) and every boundary word (3D and 2D)."""
     p = _golden(name)
     p.Nphoton = 50000
@@ -129,7 +130,8 @@ def test_tally_paths_agree_bit_for_bit(vmc, name):
                                     tally=vmc.TALLY_FIXED64, **kw)
         base = run()
         for kw in (dict(lds_slots=-1, direct_flush=1), dict(lds_slots=-1), dict(direct_flush=1),
-                   dict(lds_slots=7), dict(block_size=64), dict(block_size=64, lds_slots=-1)):
+                   dict(lds_slots=7), dict(block_size=64), dict(block_size=64, lds_slots=-1),
+                   dict(cache_epoch=1), dict(cache_epoch=5, lds_slots=7), dict(cache_epoch=3, block_size=128)):
             r = run(**kw)
             for k in OUT:
                 assert np.array_equal(getattr(r, k), getattr(base, k)), (kw, k)

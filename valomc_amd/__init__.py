@@ -54,7 +54,7 @@ class RunDesc(C.Structure):
         ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
         ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("direct_flush", C.c_int32),
-        ("chunk", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("chunk", C.c_int32), ("cache_epoch", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -174,14 +174,15 @@ class RunResult:
 def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
                   precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
                   blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-                  lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0) -> RunDesc:
+                  lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0,
+                  cache_epoch: int = 0) -> RunDesc:
     """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
     rd = RunDesc()
     rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
     rd.precision, rd.tally = int(precision), int(tally)
     rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
     rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
-    rd.direct_flush, rd.chunk = int(direct_flush), int(chunk)
+    rd.direct_flush, rd.chunk, rd.cache_epoch = int(direct_flush), int(chunk), int(cache_epoch)
     return rd
 
 
@@ -296,12 +297,12 @@ class Problem:
     def run(self, nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
-            lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0,
+            lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0, cache_epoch: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
-                           direct_flush=direct_flush, chunk=chunk)
+                           direct_flush=direct_flush, chunk=chunk, cache_epoch=cache_epoch)
         out, o = self._outputs()
         s = Stats()
         cb = None
@@ -319,7 +320,7 @@ class Problem:
     def autotune(self, nphoton: int, trial_photons: int = 0, **run_kwargs) -> dict:
         """vmc_problem_autotune: the launch shape and tally strategy that run
         ``run(nphoton, **run_kwargs)`` fastest, as keyword arguments for
-        ``run`` (block_size, refill_min, chunk, replicas, lds_slots), plus
+        ``run`` (block_size, refill_min, chunk, replicas, lds_slots, cache_epoch), plus
         ``trials``, the number of timed trial runs."""
         rd = make_run_desc(nphoton, **run_kwargs)
         tuned = RunDesc()
@@ -329,7 +330,8 @@ class Problem:
         if st:
             raise VmcError(st, self.lib)
         return dict(block_size=tuned.block_size, refill_min=tuned.refill_min, chunk=tuned.chunk,
-                    replicas=tuned.replicas, lds_slots=tuned.lds_slots, trials=int(n.value))
+                    replicas=tuned.replicas, lds_slots=tuned.lds_slots, cache_epoch=tuned.cache_epoch,
+                trials=int(n.value))
 
 
 def _batch_array(items, common):
@@ -372,7 +374,7 @@ def batch_autotune(items, trial_fraction: float = 0.0, **common) -> dict:
     """vmc_batch_autotune: launch knobs for ``batch_run(items, **common)``
     timed on the same batch with ``trial_fraction`` of its photons (0 = 2 %);
     returns keyword arguments (block_size, refill_min, chunk, replicas,
-    lds_slots) to add to ``common``, plus ``trials``."""
+    lds_slots, cache_epoch) to add to ``common``, plus ``trials``."""
     lib = load_library()
     arr, _ = _batch_array(items, common)
     tuned = RunDesc()
@@ -381,7 +383,8 @@ def batch_autotune(items, trial_fraction: float = 0.0, **common) -> dict:
     if st:
         raise VmcError(st, lib)
     return dict(block_size=tuned.block_size, refill_min=tuned.refill_min, chunk=tuned.chunk,
-                replicas=tuned.replicas, lds_slots=tuned.lds_slots, trials=int(n.value))
+                replicas=tuned.replicas, lds_slots=tuned.lds_slots, cache_epoch=tuned.cache_epoch,
+                trials=int(n.value))
 
 
 class BatchPlan:

@@ -68,6 +68,8 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   if (run->chunk != 0 && (run->chunk < 64 || run->chunk > 4096 || (run->chunk & (run->chunk - 1))))
     return VMC_ERR_INVALID_ARGUMENT;
   c->chunk = run->chunk ? (uint64_t)run->chunk : kChunk;
+  if (run->cache_epoch < -1) return VMC_ERR_INVALID_ARGUMENT;
+  c->epoch = run->cache_epoch;
   return VMC_OK;
 }
 
@@ -151,6 +153,9 @@ WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<I
   P.xcc_replicas = c.xcc ? 1 : 0;
   P.lds_slots = c.slots;
   P.direct_flush = c.direct ? 1 : 0;
+  // a batch's items follow one another through the launch, so the cache
+  // slots are handed back every 1024 passes for the items walked next
+  P.cache_epoch = c.epoch > 0 ? c.epoch : (c.epoch == 0 && w.batch && c.slots > 0 ? 1024 : 0);
   P.weight0 = 0.001;   // MC3D.hpp:182
   P.chance = 0.1;      // MC3D.hpp:183
   P.weight0_f = (float)P.weight0;
@@ -298,6 +303,7 @@ int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trial
       {&vmc_run_desc::chunk, {128, 64, 256, 512}},
       {&vmc_run_desc::replicas, {0, 8, 1}},
       {&vmc_run_desc::lds_slots, {0, -1}},
+      {&vmc_run_desc::cache_epoch, {0, -1, 256, 4096}},
   };
   for (const Knob &k : knobs) {
     for (int32_t v : k.values) {

@@ -210,6 +210,7 @@ struct LaunchCfg {
   int W = 1, block = 256, slots = 0, refill = 6, nrep = 8;
   bool direct = false;
   uint64_t chunk = kChunk;
+  int epoch = 0;   // -1: never; 0: by launch (batches 1024); > 0: passes
   size_t lds = 0;
 };
 

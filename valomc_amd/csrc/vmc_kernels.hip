@@ -547,6 +547,9 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   for (int i = threadIdx.x; i < S * WS; i += blockDim.x) cache.val[i] = 0ull;
 #pragma unroll
   for (int j = 0; j < WS; ++j) sbuf[j * bstride] = R(0);
+  // cache epochs: waves that have finished, counted in LDS
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = 0;
   __syncthreads();
   uint64_t steps = 0;   // wave-uniform
   // The lane's current sector.  FP64 tallies: a sector with a cache slot
@@ -557,6 +560,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   // whichever sectors hold slots.
   int sect = -1;
   int cslot = -1;
+  const int epoch = S > 0 ? P.cache_epoch : 0;   // wave-uniform
+  const int nwaves = (int)(blockDim.x >> 6);
+  int pass = 0;            // wave-uniform: passes since the last epoch
+  bool wave_done = false;  // wave-uniform: no photons left and none to come
 
   // Add to the replica: element word el*W + w (`etally`) or boundary word
   // ib*4 + j (`btally`).  With one replica per XCD (`local`), every add to a
@@ -709,7 +716,20 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       }
     }
     const uint64_t live = __ballot(alive);
-    if (live == 0) break;
+    if (live == 0) {
+      // with cache epochs a finished wave keeps passing through the loop
+      // (doing nothing) until the whole workgroup leaves it together at an
+      // epoch; a wave whose refill drew only the padding of an item's last
+      // chunk is not finished while the dispenser still has ids
+      if (epoch == 0) break;
+      if (exhausted && !wave_done) {
+        wave_done = true;
+        if (lane == 0) atomicAdd(&s_done, 1);
+      }
+      // a finished wave goes straight to the next epoch and waits in its
+      // barrier instead of taking issue slots with empty passes
+      if (wave_done) pass = epoch - 1;
+    }
     steps += __popcll(live);
 
     // ---- one element step (returns early where the reference's loop does) ----
@@ -1022,6 +1042,35 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       }
       pe_el = -1;
       pb_ib = -1;
+    }
+
+    // ---- cache epoch: every `epoch` passes the workgroup moves the slots'
+    // sums to its replica and frees the slots, so the sectors of the items
+    // walked now (a batch's later items) can claim them.  Every wave makes
+    // the same passes between two epochs (a finished wave skips to the next
+    // epoch) and leaves the loop only here, all together, once every wave
+    // has finished (the count is read between barriers, where no wave
+    // writes it). ----
+    if (epoch > 0 && ++pass == epoch) {
+      pass = 0;
+      __syncthreads();
+      const bool all_done = *(volatile int *)&s_done == nwaves;
+      if (!all_done) {
+        for (int i = threadIdx.x; i < S * WS; i += blockDim.x) {
+          const int sc = cache.tag[i / WS];
+          const unsigned long long bits = cache.val[i];
+          if (sc >= 0 && bits) replica_add(etally, (int64_t)sc * WS + i % WS, bits);
+          cache.val[i] = 0ull;
+        }
+      }
+      __syncthreads();
+      if (all_done) break;
+      for (int i = threadIdx.x; i < S; i += blockDim.x) cache.tag[i] = -1;
+      __syncthreads();
+      if (cslot >= 0) {   // the lane's sector sums went with its slot
+        cslot = -1;
+        sect = -1;
+      }
     }
   }
   flush_sector(sect >= 0 && cslot < 0);

@@ -146,6 +146,7 @@ struct WalkParams {
   int32_t xcc_replicas;         // 1: replica = XCD id, workgroup-scope (L2) atomics
   int32_t lds_slots;            // LDS sector-cache slots per workgroup (0 = none)
   int32_t direct_flush;         // 1: each lane adds its own sector words (no coalescing)
+  int32_t cache_epoch;          // loop passes between sector-cache hand-overs (0: never)
   double weight0, chance;
   float weight0_f, chance_f;    // the same in FP32, for the FP32 walk
   double fix_scale;             // fixed-point scale (2^32)
