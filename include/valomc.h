@@ -74,7 +74,8 @@ typedef struct vmc_run_desc {
   int32_t precision;     /* VMC_FP64 | VMC_FP32                                */
   int32_t tally;         /* VMC_TALLY_FP64 | VMC_TALLY_FIXED64                 */
   int32_t block_size;    /* 0 = default (FP32 1024, FP64 256); multiple of 64   */
-  int32_t blocks_per_cu; /* 0 = as many as fit                                 */
+  int32_t blocks_per_cu; /* resident workgroups per CU: 0 = as many as fit,
+                            else at most this many                          */
   int32_t replicas;      /* tally copies: > 0 that many (device-scope atomics);
                             0 = automatic: one per XCD (8, L2 atomics), or a
                             single one when a copy is 4..64 MiB             */

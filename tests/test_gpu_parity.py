@@ -152,6 +152,10 @@ def test_results_do_not_depend_on_launch_shape(vmc):
     with vmc.Problem(p) as prob:
         a = prob.run(p.Nphoton, seed=9, precision=vmc.FP64)
         b = prob.run(p.Nphoton, seed=9, precision=vmc.FP64, block_size=64, replicas=1, refill_min=32)
+        c = prob.run(p.Nphoton, seed=9, precision=vmc.FP64, blocks_per_cu=1)
+        assert c.grid_blocks == a.grid_blocks // 2 and c.grid_blocks > 0   # FP64: two 256-lane groups per CU
+        for k in OUT:
+            assert rel_l2(getattr(a, k), getattr(c, k)) < 1e-12, k
     for k in OUT:
         assert rel_l2(getattr(a, k), getattr(b, k)) < 1e-12, k
 

@@ -70,6 +70,8 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   c->chunk = run->chunk ? (uint64_t)run->chunk : kChunk;
   if (run->cache_epoch < -1) return VMC_ERR_INVALID_ARGUMENT;
   c->epoch = run->cache_epoch;
+  if (run->blocks_per_cu < 0) return VMC_ERR_INVALID_ARGUMENT;
+  c->bpc = run->blocks_per_cu;
   return VMC_OK;
 }
 
@@ -132,7 +134,7 @@ WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<I
   int bpc = 0;
   hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, w.batch, c.block, c.lds, &bpc), "occupancy");
   if (bpc < 1) throw HipFail{hipErrorInvalidConfiguration, "workgroup does not fit on a CU (block size / LDS tally cache)"};
-  w.grid = bpc * num_cus;
+  w.grid = (c.bpc > 0 ? std::min(bpc, c.bpc) : bpc) * num_cus;
   hip_check(hipMemcpyAsync(a.items.p, items.data(), items.size() * sizeof(ItemRec), hipMemcpyHostToDevice, a.stream),
             "upload");
   WalkParams &P = w.P;

@@ -211,6 +211,7 @@ struct LaunchCfg {
   bool direct = false;
   uint64_t chunk = kChunk;
   int epoch = 0;   // -1: never; 0: by launch (batches 1024); > 0: passes
+  int bpc = 0;     // workgroups per CU: 0 = as many as fit, else at most this many
   size_t lds = 0;
 };
 
