@@ -87,10 +87,15 @@ def copy_reference(nbytes: int):
                       "launches": 3}), flush=True)
 
 
-def traffic_per_step():
-    """HBM bytes per element step of the FP32 CW walk, from the committed
-    FETCH_SIZE / WRITE_SIZE profile of this workload (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "walk3d_fp32_box384k_hbm_traffic.txt")
+TRAFFIC_PROFILES = {"box": "walk3d_fp32_box384k_hbm_traffic.txt", "fd": "walk3d_fp32fd_box2m_hbm_traffic.txt"}
+
+
+def traffic_per_step(workload="box"):
+    """HBM bytes per element step of the workload's FP32 walk, from its
+    committed FETCH_SIZE / WRITE_SIZE profile (None if absent)."""
+    if workload not in TRAFFIC_PROFILES:
+        return None
+    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES[workload])
     try:
         with open(path) as fh:
             for line in fh:
@@ -332,7 +337,7 @@ def main():
     steps_total = sum(r.steps for r in runs)
     bstep = B_STEP[("fp32", fd)] if p.dim == 3 else B_STEP_2D
     achieved = steps_total * bstep / kernel_s / 1e9
-    tps = traffic_per_step() if a.workload == "box" else None
+    tps = traffic_per_step(a.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if tps is None else round(tps * steps_total / a.steps),
