@@ -183,6 +183,9 @@ static int cli_main(int argc, char **argv) {
     printf("Error while loading!\n");
     return 1;
   }
+  // optional sections the file held (Init() fills in defaults)
+  const int had_bcn = MC.BCn.N > 0, had_bcl = MC.BCLNormal.N > 0, had_int = MC.BCIntensity.N > 0,
+            had_sigma = MC.GaussianSigma.N > 0;
   printf("Initializing %s\n", name);
   try {
     MC.ErrorChecks();
@@ -195,7 +198,9 @@ static int cli_main(int argc, char **argv) {
     return 1;
   }
   if (check_only) {
-    // parse + ErrorChecks + Init only: a summary line, no transport
+    // parse + ErrorChecks + Init only: a summary line (sizes, constants, a
+    // checksum of the neighbour table, the optional sections read), no
+    // transport
     long long hsum = 0;
     for (int_fast64_t i = 0; i < MC.HN.N; ++i) hsum += (long long)(MC.HN[i] % 1000003);
     int nls = 0, ndir = 0;
@@ -204,8 +209,7 @@ static int cli_main(int argc, char **argv) {
     printf("CHECK dim=%d Ne=%ld Nb=%ld Nr=%ld Nphoton=%ld seed=%lu f=%.10g phase0=%.10g hnsum=%lld "
            "nonabsorbing_bc=%d directed=%d bcn=%d bcl=%d intensity=%d sigma=%d\n",
            DIM, (long)MC.H.Nx, (long)MC.BH.Nx, (long)MC.r.Nx, (long)MC.Nphoton, MC.seed, MC.f, MC.phase0,
-           hsum, nls, ndir, (int)(MC.BCn.N > 0), (int)(MC.BCLNormal.N > 0), (int)(MC.BCIntensity.N > 0),
-           (int)(MC.GaussianSigma.N > 0));
+           hsum, nls, ndir, had_bcn, had_bcl, had_int, had_sigma);
     return 0;
   }
   printf("Computing...\n");
