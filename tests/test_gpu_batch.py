@@ -176,14 +176,15 @@ def test_many_items_equal_single_runs(vmc):
         probq.close()
 
 
-def test_batch_plan_replays_equal_eager_batches(vmc):
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_plan_replays_equal_eager_batches(vmc, precision):
     """A batch recorded once as a HIP graph (2D and 3D, CW and FD groups, an
     empty item) and replayed with new seeds, with eager launches between the
     replays: every replay equals the eager batch of the same seeds bit for
     bit, a medium changed between replays is used by the next one, and
     items that do not match the plan are refused."""
     probs, items = _mix(vmc)
-    kw_common = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+    kw_common = dict(precision=vmc.FP32 if precision == "fp32" else vmc.FP64, tally=vmc.TALLY_FIXED64)
     try:
         with vmc.BatchPlan(items, **kw_common) as plan:
             for rep in range(3):

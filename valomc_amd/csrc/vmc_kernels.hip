@@ -802,7 +802,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       if constexpr (D == 3) asm volatile("" ::"v"(nb.w));
 #pragma unroll
       for (int f = 0; f < NF; ++f) asm volatile("" ::"v"(nbo[f].x), "v"(nbo[f].y));
-      if (out_cur || bf < 0 || p.nsteps > P.step_cap) {
+      if (out_cur || bf < 0 || (uint32_t)p.nsteps > P.step_cap) {
         // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
         lose();
         return;
