@@ -52,14 +52,15 @@ def parse():
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
-    ap.add_argument("--requests", type=int, default=300, help="serve workload: number of requests")
+    ap.add_argument("--requests", type=int, default=1000, help="serve workload: number of requests")
     ap.add_argument("--utilization", type=float, default=0.8,
                     help="serve workload: arrival rate as a fraction of the measured service rate")
     ap.add_argument("--workload", choices=("box", "fd", "square2d", "batch", "serve"), default="box",
                     help="box: the 384k-tet box (the headline line); batch: 256 instances "
                          "(64 pencil positions x 4 parameter sets) of the same box in one launch")
-    ap.add_argument("--photon-scale", type=float, default=0.1,
-                    help="batch workload: photon counts are log-uniform in [1e6, 1e8] times this")
+    ap.add_argument("--photon-scale", type=float, default=1.0,
+                    help="batch workload: photon counts are log-uniform in [1e6, 1e8] times this "
+                         "(1.0: about 5.5e9 photons per batch)")
     ap.add_argument("--autotune", action="store_true",
                     help="box/fd/square2d: pick block size, refill threshold, photons per wave fetch and "
                          "tally strategy with vmc_problem_autotune before the timed steps (tuning time "
@@ -161,6 +162,7 @@ def run_batch(a):
               tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
     small = [(pr, dict(nphoton=max(1, k["nphoton"] // 100), seed=k["seed"])) for pr, k in items]
     vmc.batch_run(small, **kw)   # warm-up
+    print(f"batch: {len(items)} items, {total:.3e} photons; autotune", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     tuned = vmc.batch_autotune(items, **kw)
     tuned["tuning_s"] = round(time.perf_counter() - t0, 3)
@@ -170,18 +172,21 @@ def run_batch(a):
     t0 = time.perf_counter()
     plan = vmc.BatchPlan(items, **kw)
     plan_s = time.perf_counter() - t0
+    print(f"batch: knobs {tuned}; graph warm-up replay", file=sys.stderr, flush=True)
     plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
     walls = []
-    for _ in range(a.steps):
+    for i in range(a.steps):
         t0 = time.perf_counter()
         res, st = plan.run()
         walls.append(time.perf_counter() - t0)
+        print(f"batch: replay {i + 1}/{a.steps} {walls[-1]:.2f} s", file=sys.stderr, flush=True)
     wall = sum(walls) / len(walls)
     plan.close()
     # the same batch launched eagerly
     t0 = time.perf_counter()
     eres, est = vmc.batch_run(items, **kw)
     eager_wall = time.perf_counter() - t0
+    print(f"batch: eager launch {eager_wall:.2f} s; one solve at a time", file=sys.stderr, flush=True)
     graph_same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, eres))
     # the same items one solve at a time
     t0 = time.perf_counter()
@@ -192,6 +197,7 @@ def run_batch(a):
         seq_kernel += r.kernel_ms
         seq.append(r)
     seq_wall = time.perf_counter() - t0
+    print(f"batch: sequential {seq_wall:.2f} s; FP64 runs of the first 4 items", file=sys.stderr, flush=True)
     same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, seq))
     # accuracy: FP32 batch items against FP64 runs of the same streams (first 4)
     rel = []
@@ -243,8 +249,20 @@ def run_serve(a):
             rate[(p.notes["kind"], k % 4, f)] = 2000000 / (r.kernel_ms / 1e3)   # device time per photon
         cache.release(prob)
     service = [q["photons"] / rate[(q["kind"], q["index"], q["f"])] for q in reqs]
-    lam = a.utilization / float(np.mean(service))
+    lam0 = a.utilization / float(np.mean(service))
+    # requests in flight together share launches, so the device is busy for
+    # less than the sum of their alone times: one correction of the rate
+    # from a calibration stream (other seeds) at the estimated rate
+    ncal = min(len(reqs), 150)
+    cal = [dict(q, seed=q["seed"] + 100000) for q in reqs[:ncal]]
+    print(f"serve: calibration stream of {ncal} requests at {lam0:.2f}/s", file=sys.stderr, flush=True)
+    with vmc.Server(arena_bytes=16 << 30, **kw) as srv:
+        _, wcal = serving.drive(srv, cache, pool, cal, lam0, seed=11)
+        busy0 = srv.counters()["busy_ms"] / 1e3 / wcal
+    lam = lam0 * min(1.5, max(2.0 / 3.0, a.utilization / max(busy0, 1e-3)))
     keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
+    print(f"serve: calibration busy {busy0:.3f}; {len(reqs)} requests at {lam:.2f}/s", file=sys.stderr,
+          flush=True)
     with vmc.Server(arena_bytes=16 << 30, **kw) as srv:
         out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
         ctr = srv.counters()
@@ -268,6 +286,7 @@ def run_serve(a):
         "config": {"workload": "serving: Poisson stream, 40% 2D (1k-200k triangles) / 60% 3D (50k-2M tets) "
                                "resident jittered meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50",
                    "requests": len(out), "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
+                   "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3)},
                    "photons_total": total, "parallelism": "single GPU"},
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
