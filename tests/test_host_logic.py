@@ -1,6 +1,8 @@
 """Host-side logic that needs no GPU: the benchmark input generators
-(batched instances, serving pool and request mix), the serving cache key,
+(createGridMesh numbering against the reference's pixel/voxel examples,
+batched instances, serving pool and request mix), the serving cache key,
 and bench.py's refusal of more than one GPU."""
+import math
 import os
 import subprocess
 import sys
@@ -82,3 +84,75 @@ def test_bench_refuses_several_gpus():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_grid_mesh_2d_follows_the_documented_numbering():
+    """createGridMesh on a 2x2 pixel grid, against the numbering drawn in
+    examples/pixeltest.m (docs/pixeltest.html): vertices 1..9 with y
+    ascending first, lower triangles t1..t4 of the pixels in meshgrid order,
+    then the upper triangles t5..t8, every pixel split along the diagonal
+    from its top-left to its bottom-right corner."""
+    r, H, BH = meshgen.grid_mesh([-0.5, 0.5], [-0.5, 0.5])
+    assert np.array_equal(r, [[x, y] for x in (-1, 0, 1) for y in (-1, 0, 1)])
+    tris = [{1, 2, 4}, {2, 3, 5}, {4, 5, 7}, {5, 6, 8}, {2, 4, 5}, {3, 5, 6}, {5, 7, 8}, {6, 8, 9}]
+    assert [set(t) for t in (H + 1).tolist()] == tris
+    # the boundary runs once round the square, clockwise from the lower
+    # left edge (createGridMesh.m:141-176)
+    assert (BH + 1).tolist() == [[1, 2], [2, 3], [3, 6], [6, 9], [9, 8], [8, 7], [7, 4], [4, 1]]
+
+
+def test_grid_mesh_2d_config_square():
+    """createRectangularMesh(10, 10, 1): 121 nodes, 200 triangles, 40 edges;
+    boundary rows 4-7 (1-based) are the edges of x = -5 with y in [-2, 2],
+    the cosinic source of the 2D benchmark square."""
+    r, H, BH = meshgen.rectangular_mesh(10, 10, 1)
+    assert (r.shape[0], H.shape[0], BH.shape[0]) == (121, 200, 40)
+    e = r[BH[3:7]]
+    assert np.all(e[:, :, 0] == -5.0)
+    assert e[:, :, 1].min() == -2.0 and e[:, :, 1].max() == 2.0
+    # each element: positive area, the mesh tiles the square
+    a = r[H]
+    area = 0.5 * np.abs((a[:, 1, 0] - a[:, 0, 0]) * (a[:, 2, 1] - a[:, 0, 1])
+                        - (a[:, 2, 0] - a[:, 0, 0]) * (a[:, 1, 1] - a[:, 0, 1]))
+    assert np.allclose(area, 0.5) and math.isclose(area.sum(), 100.0)
+
+
+def test_grid_mesh_3d_voxel_order():
+    """createGridMesh in 3D, against examples/voxeltest.m: six tetrahedra per
+    voxel; element e lies in voxel e mod nvox, voxels in meshgrid order (y
+    ascending first, then x, then z); the six tetrahedra of a voxel fill
+    it; the boundary triangles are the voxel faces on the outer surface,
+    each a face of exactly one tetrahedron."""
+    xv, yv, zv = np.arange(3) + 0.5, np.arange(4) + 0.5, np.arange(2) + 0.5
+    r, H, BH = meshgen.grid_mesh(xv, yv, zv)
+    nx, ny, nz = len(xv), len(yv), len(zv)
+    nvox = nx * ny * nz
+    assert H.shape == (6 * nvox, 4) and r.shape == ((nx + 1) * (ny + 1) * (nz + 1), 3)
+    cen = r[H].mean(axis=1)
+    ix, iy, iz = (np.floor(cen[:, d]).astype(int) for d in range(3))
+    v = np.arange(6 * nvox) % nvox
+    assert np.array_equal(iy + ny * ix + ny * nx * iz, v)
+    t = r[H]
+    vol = np.abs(np.einsum("ij,ij->i", t[:, 1] - t[:, 0], np.cross(t[:, 2] - t[:, 0], t[:, 3] - t[:, 0]))) / 6
+    assert np.allclose(np.bincount(v, weights=vol), 1.0)
+    assert BH.shape[0] == 4 * (nx * ny + ny * nz + nz * nx)
+    b = r[BH]
+    on_surface = np.zeros(BH.shape[0], dtype=bool)
+    for d, hi in ((0, nx), (1, ny), (2, nz)):
+        on_surface |= np.all(b[:, :, d] == 0, axis=1) | np.all(b[:, :, d] == hi, axis=1)
+    assert on_surface.all()
+    faces = {}
+    for e, row in enumerate(H.tolist()):
+        for f in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+            faces.setdefault(tuple(sorted(row[i] for i in f)), []).append(e)
+    assert all(len(faces.get(tuple(sorted(row)), [])) == 1 for row in BH.tolist())
+
+
+def test_pencil_face_of_the_box():
+    """The 384k-tet box: 68,921 nodes, 19,200 boundary triangles; the pencil
+    at (0.3, 0.2) on the top face lands on boundary row 3282 (0-based)."""
+    p = meshgen.sphere_box_3d(nphoton=1)
+    assert (p.Ne, p.Nb, p.Nr) == (384000, 19200, 68921)
+    assert p.notes["pencil_face"] == 3282
+    tri = np.asarray(p.r)[np.asarray(p.BH)[3282]]
+    assert np.all(tri[:, 2] == 20.0)
