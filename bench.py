@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
     ap.add_argument("--requests", type=int, default=1000, help="serve workload: number of requests")
+    ap.add_argument("--policy", choices=("fair", "shortest"), default="shortest",
+                    help="serve workload: the server's scheduling policy")
     ap.add_argument("--utilization", type=float, default=0.8,
                     help="serve workload: arrival rate as a fraction of the measured service rate")
     ap.add_argument("--workload", choices=("box", "fd", "square2d", "batch", "serve"), default="box",
@@ -239,6 +241,7 @@ def run_serve(a):
     reqs = meshgen.serving_requests(a.requests, seed=2025)
     cache = serving.MeshCache(budget_bytes=16 << 30)
     kw = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
+    policy = vmc.SERVE_SHORTEST if a.policy == "shortest" else vmc.SERVE_FAIR
     # service rate per (mesh, mode) from short single runs
     rate = {}
     for k, p in enumerate(pool):
@@ -256,14 +259,14 @@ def run_serve(a):
     ncal = min(len(reqs), 150)
     cal = [dict(q, seed=q["seed"] + 100000) for q in reqs[:ncal]]
     print(f"serve: calibration stream of {ncal} requests at {lam0:.2f}/s", file=sys.stderr, flush=True)
-    with vmc.Server(arena_bytes=16 << 30, **kw) as srv:
+    with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
         _, wcal = serving.drive(srv, cache, pool, cal, lam0, seed=11)
         busy0 = srv.counters()["busy_ms"] / 1e3 / wcal
     lam = lam0 * min(1.5, max(2.0 / 3.0, a.utilization / max(busy0, 1e-3)))
     keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
     print(f"serve: calibration busy {busy0:.3f}; {len(reqs)} requests at {lam:.2f}/s", file=sys.stderr,
           flush=True)
-    with vmc.Server(arena_bytes=16 << 30, **kw) as srv:
+    with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
         out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
         ctr = srv.counters()
     lat = np.array([o["latency_ms"] for o in out])
@@ -287,7 +290,7 @@ def run_serve(a):
                                "resident jittered meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50",
                    "requests": len(out), "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
                    "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3)},
-                   "photons_total": total, "parallelism": "single GPU"},
+                   "photons_total": total, "policy": a.policy, "parallelism": "single GPU"},
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
         "latency_ms_max": round(float(lat.max()), 2),

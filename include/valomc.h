@@ -252,15 +252,24 @@ int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr,
    reference; for callers that issue many MonteCarlo() solves as they come
    (an inverse-problem loop, a request queue).  A worker thread owns the GPU
    work.  Requests are admitted in arrival order when their tallies fit in the
-   server's device arena; every launch advances all admitted requests of one
-   (dimension, CW/FD) group by a share of `launch_photons`, so requests that
-   arrive while others run join at the next launch.  Each request's photons
+   server's device arena; every launch advances admitted requests of one
+   (dimension, CW/FD) group by parts of `launch_photons` (the scheduling
+   policy below decides which and how much), so requests that arrive while
+   others run join at the next launch.  Each request's photons
    keep their stream ids (seed, stream_tag, photon index), so its result equals
    vmc_problem_run's for the same inputs (bit for bit with
    VMC_TALLY_FIXED64).  A problem and its output buffers belong to the server
    from submit until wait returns for that ticket.
    ------------------------------------------------------------------------ */
 typedef struct vmc_server vmc_server;
+
+#define VMC_SERVE_SHORTEST 0 /* least estimated remaining device time first:
+                                the launch's group is that request's, and the
+                                budget goes to the group's requests in order
+                                of remaining time (time per photon measured
+                                per problem from earlier launches)           */
+#define VMC_SERVE_FAIR 1     /* groups in turn; equal shares of a launch for
+                                the group's admitted requests               */
 
 typedef struct vmc_server_desc {
   int32_t precision;      /* VMC_FP64 | VMC_FP32, for every request              */
@@ -270,7 +279,8 @@ typedef struct vmc_server_desc {
   int64_t launch_photons; /* photons per launch, shared by the requests in it;
                              0 = 2e7 (about 0.15 s of FP32 work)                  */
   int64_t min_slice;      /* least photons a request gets in a launch; 0 = 262144 */
-  int32_t reserved[8];
+  int32_t policy;         /* VMC_SERVE_SHORTEST (0) | VMC_SERVE_FAIR             */
+  int32_t reserved[7];
 } vmc_server_desc;
 
 int vmc_server_create(const vmc_server_desc *desc, vmc_server **out);

@@ -3,7 +3,8 @@
 A request's photons keep their stream ids however the server slices them
 over launches and whatever shares the launches, so in fixed-point mode each
 result equals vmc_problem_run's for the same inputs bit for bit: alone, in a
-mix submitted together, and when submitted while others are running.  A
+mix submitted together, and when submitted while others are running, under
+both scheduling policies (fair shares, shortest remaining time first).  A
 small arena forces requests to queue for admission.
 """
 import os
@@ -47,8 +48,9 @@ def _problems(vmc):
     return probs
 
 
-@pytest.mark.parametrize("arena", [0, 24 << 20])
-def test_server_results_equal_single_runs(vmc, arena):
+@pytest.mark.parametrize("arena,policy", [(0, "SERVE_SHORTEST"), (24 << 20, "SERVE_SHORTEST"),
+                                          (0, "SERVE_FAIR"), (24 << 20, "SERVE_FAIR")])
+def test_server_results_equal_single_runs(vmc, arena, policy):
     probs = _problems(vmc)
     try:
         singles = []
@@ -60,7 +62,7 @@ def test_server_results_equal_single_runs(vmc, arena):
         # small launches so requests are sliced; the second half arrives while
         # the first half runs
         with vmc.Server(precision=vmc.FP32, tally=vmc.TALLY_FIXED64, arena_bytes=arena,
-                        launch_photons=200000, min_slice=50000) as srv:
+                        launch_photons=200000, min_slice=50000, policy=getattr(vmc, policy)) as srv:
             tickets = []
             for k, (name, kw) in enumerate(REQS):
                 p, prob = probs[name]
@@ -88,3 +90,8 @@ def test_server_refuses_request_larger_than_arena(vmc):
         with pytest.raises(vmc.VmcError) as ei:
             srv.submit(prob, 1000)
         assert ei.value.status == 101
+
+
+def test_server_policy_is_validated(vmc):
+    with pytest.raises(vmc.VmcError):
+        vmc.Server(policy=2)

@@ -23,6 +23,8 @@ FP64 = 0
 FP32 = 1
 TALLY_FP64 = 0
 TALLY_FIXED64 = 1
+SERVE_SHORTEST = 0   # server policy: least estimated remaining device time first
+SERVE_FAIR = 1       # server policy: groups in turn, equal shares of a launch
 
 # reference mcerror order (cpp/Errors.hpp:4-21); status = 1 + code
 MCERRORS = (
@@ -434,7 +436,8 @@ class BatchPlan:
 
 class ServerDesc(C.Structure):
     _fields_ = [("precision", C.c_int32), ("tally", C.c_int32), ("arena_bytes", C.c_int64),
-                ("launch_photons", C.c_int64), ("min_slice", C.c_int64), ("reserved", C.c_int32 * 8)]
+                ("launch_photons", C.c_int64), ("min_slice", C.c_int64), ("policy", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
 
 class Server:
@@ -444,9 +447,10 @@ class Server:
     completion)."""
 
     def __init__(self, precision: int = FP32, tally: int = TALLY_FP64, arena_bytes: int = 0,
-                 launch_photons: int = 0, min_slice: int = 0):
+                 launch_photons: int = 0, min_slice: int = 0, policy: int = SERVE_SHORTEST):
         self.lib = load_library()
-        d = ServerDesc(int(precision), int(tally), int(arena_bytes), int(launch_photons), int(min_slice))
+        d = ServerDesc(int(precision), int(tally), int(arena_bytes), int(launch_photons), int(min_slice),
+                       int(policy))
         h = C.c_void_p()
         st = self.lib.vmc_server_create(C.byref(d), C.byref(h))
         if st:

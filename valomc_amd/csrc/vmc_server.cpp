@@ -18,9 +18,13 @@ using namespace vmc;
 // A worker thread owns one stream and a persistent tally arena.  Requests
 // wait in FIFO order until their tallies fit in the arena (admission by
 // device-memory footprint); each launch then takes one (dimension, CW/FD)
-// group of the admitted requests and advances every request in it by a
-// share of the launch's photon budget, so a request that arrives while
-// others run joins at the next launch and launch times stay bounded.  A
+// group of the admitted requests and advances them by parts of the
+// launch's photon budget, so a request that arrives while others run joins
+// at the next launch and launch times stay bounded.  Fair policy: groups in
+// turn, equal shares.  Shortest policy: the group of the request with the
+// least estimated remaining device time, whose requests take the budget in
+// order of remaining time (each problem's device time per photon is
+// measured from the launches it took part in).  A
 // request's photons keep their stream ids (seed, tag, photon index) however
 // they are sliced, so its result does not depend on what it shared the GPU
 // with: bit for bit in fixed-point mode.
@@ -99,6 +103,9 @@ struct vmc_server {
   int64_t launches = 0, photons = 0;
   double busy_ms = 0;
   uint64_t epoch = 0;   // bumped by every submit and every release of arena ranges
+  // device ms per photon of each problem, from the launches it took part in
+  // (shortest policy; a scheduling estimate only)
+  std::map<const vmc_problem *, double> cost;
   std::thread worker;
   // finisher: reduces, copies and normalises completed requests on its own
   // stream while the worker keeps launching, then returns their ranges
@@ -161,6 +168,42 @@ void finisher_loop(vmc_server *s) {
   }
 }
 
+// Shortest policy: the active requests with photons left, of the group
+// (dimension, CW/FD) of the one with the least estimated remaining device
+// time, in order of that estimate (ties: earlier ticket).  A problem never
+// launched yet is estimated at its group's least known cost per photon, or
+// 0, so a new request is measured in its first launch.  Called under the
+// server's lock.
+void pick_shortest(vmc_server *s, std::vector<Request *> &group) {
+  struct Est {
+    double t;
+    Request *r;
+  };
+  std::vector<Est> all;
+  double gmin[4] = {-1, -1, -1, -1};
+  auto gid = [](const Request *r) { return (r->prob->dim == 3 ? 0 : 2) + (r->fd ? 1 : 0); };
+  for (Request *r : s->active) {
+    auto it = s->cost.find(r->prob);
+    if (it == s->cost.end()) continue;
+    double &g = gmin[gid(r)];
+    g = g < 0 ? it->second : std::min(g, it->second);
+  }
+  for (Request *r : s->active) {
+    const int64_t left = r->run.Nphoton - r->launched;
+    if (left <= 0) continue;
+    auto it = s->cost.find(r->prob);
+    const double c = it != s->cost.end() ? it->second : std::max(0.0, gmin[gid(r)]);
+    all.push_back({c * (double)left, r});
+  }
+  if (all.empty()) return;
+  std::sort(all.begin(), all.end(), [](const Est &a, const Est &b) {
+    return a.t != b.t ? a.t < b.t : a.r->ticket < b.r->ticket;
+  });
+  const int g0 = gid(all[0].r);
+  for (const Est &e : all)
+    if (gid(e.r) == g0) group.push_back(e.r);
+}
+
 void server_loop(vmc_server *s) {
   (void)hipSetDevice(s->device);
   uint64_t seen = ~0ull;   // epoch of the last admission attempt that found no room
@@ -195,18 +238,26 @@ void server_loop(vmc_server *s) {
         seen = s->epoch;   // nothing fits until the finisher returns ranges
         continue;
       }
-      // the next group in turn among the (dimension, CW/FD) groups with work
-      for (int k = 0; k < 4; ++k) {
-        const int g = (s->turn + k) % 4;
-        const int dim = g < 2 ? 3 : 2;
-        const bool fd = (g & 1) != 0;
-        for (Request *r : s->active)
-          if (r->prob->dim == dim && r->fd == fd && r->launched < r->run.Nphoton) group.push_back(r);
+      if (s->desc.policy == VMC_SERVE_SHORTEST) {
+        pick_shortest(s, group);
         if (!group.empty()) {
-          gdim = dim;
-          gfd = fd;
-          s->turn = (g + 1) % 4;
-          break;
+          gdim = group[0]->prob->dim;
+          gfd = group[0]->fd;
+        }
+      } else {
+        // the next group in turn among the (dimension, CW/FD) groups with work
+        for (int k = 0; k < 4; ++k) {
+          const int g = (s->turn + k) % 4;
+          const int dim = g < 2 ? 3 : 2;
+          const bool fd = (g & 1) != 0;
+          for (Request *r : s->active)
+            if (r->prob->dim == dim && r->fd == fd && r->launched < r->run.Nphoton) group.push_back(r);
+          if (!group.empty()) {
+            gdim = dim;
+            gfd = fd;
+            s->turn = (g + 1) % 4;
+            break;
+          }
         }
       }
     }
@@ -226,17 +277,27 @@ void server_loop(vmc_server *s) {
       if (!group.empty()) {
         const int st = resolve_cfg(gdim, s->num_cus, s->lds_per_cu, gfd, &tmpl, &c);
         if (st != VMC_OK) throw HipFail{hipErrorInvalidValue, "server launch configuration"};
-        // fair shares of the launch budget
+        // fair: equal shares of the launch budget; shortest: the budget in
+        // the group's order (least remaining time first), at least
+        // min_slice for the first
         const int64_t share = std::max(s->desc.min_slice, s->desc.launch_photons / (int64_t)group.size());
+        int64_t budget = std::max(s->desc.min_slice, s->desc.launch_photons);
         std::vector<ItemRec> items;
         std::vector<int64_t> slice;
         for (Request *r : group) {
-          const int64_t n = std::min(share, r->run.Nphoton - r->launched);
+          const int64_t left = r->run.Nphoton - r->launched;
+          int64_t n = std::min(share, left);
+          if (s->desc.policy == VMC_SERVE_SHORTEST) {
+            if (budget <= 0) break;
+            n = std::min(budget, left);
+            budget -= n;
+          }
           upload_params(*r->prob, c.f64);
           items.push_back(make_item(*r->prob, &r->run, c.f64, (uint64_t)n, (uint64_t)r->launched, r->eoff / c.W,
                                     r->boff));
           slice.push_back(n);
         }
+        group.resize(items.size());   // the shortest policy may leave requests out
         a.ne = s->cap_words / c.W;
         a.nb = s->cap_b;
         a.nitems = items.size();
@@ -251,6 +312,13 @@ void server_loop(vmc_server *s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->launches++;
         s->busy_ms += ms;
+        int64_t nl = 0;
+        for (int64_t n : slice) nl += n;
+        if (nl > 0)
+          for (Request *r : group) {
+            double &est = s->cost[r->prob];
+            est = est > 0 ? 0.5 * est + 0.5 * ms / (double)nl : ms / (double)nl;
+          }
         for (size_t k = 0; k < group.size(); ++k) {
           Request *r = group[k];
           r->launched += slice[k];
@@ -295,6 +363,7 @@ int vmc_server_create(const vmc_server_desc *d, vmc_server **out) {
   *out = nullptr;
   if (d->precision != VMC_FP64 && d->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
   if (d->tally != VMC_TALLY_FP64 && d->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
+  if (d->policy != VMC_SERVE_FAIR && d->policy != VMC_SERVE_SHORTEST) return VMC_ERR_INVALID_ARGUMENT;
   std::unique_ptr<vmc_server> s(new vmc_server());
   s->desc = *d;
   if (s->desc.arena_bytes <= 0) s->desc.arena_bytes = (int64_t)4 << 30;
