@@ -162,11 +162,12 @@ static int cli_main(int argc, char **argv) {
   const char *name = DIM == 3 ? "MC3D" : "MC2D";
   printf("                 ValoMC-%dD (%s)\n", DIM, vmc_version());
   if (argc < 3) {
-    printf("Use syntax:\n %s inputfile outputfile [-s] [--fp32] [--fixed] [--precise]\n", name);
+    printf("Use syntax:\n %s inputfile outputfile [-s] [--fp32] [--fixed] [--precise] [--check]\n", name);
     printf("  -s        suppress the progress output\n");
     printf("  --fp32    single-precision transport (FP64 tallies)\n");
     printf("  --fixed   64-bit fixed-point tallies (bit-reproducible)\n");
     printf("  --precise write the solution with 17 significant digits\n");
+    printf("  --check   parse the file and run ErrorChecks() + Init() only (no GPU)\n");
     return 0;
   }
   MCEngine<DIM> MC;
