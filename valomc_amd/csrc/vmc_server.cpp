@@ -314,6 +314,7 @@ void server_loop(vmc_server *s) {
         s->busy_ms += ms;
         int64_t nl = 0;
         for (int64_t n : slice) nl += n;
+        if (s->cost.size() > 4096) s->cost.clear();   // estimates of problems long gone
         if (nl > 0)
           for (Request *r : group) {
             double &est = s->cost[r->prob];
