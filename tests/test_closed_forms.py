@@ -11,6 +11,12 @@ The same cases run on the oracle (CPU) and on the GPU path (FP64 and FP32).
   e^(-mua L) of the launched energy, the absorbed energy 1 - e^(-mua L), and
   nothing leaves through the other faces.  Deterministic, so the bound is
   rounding.
+* Frequency domain: the same slab at 5 GHz with phase0 = 0.3.  A photon
+  reaches the bottom with weight e^(-mua L) and phase phase0 + k L
+  (k = 2 pi f n / c0), so the bottom's EBR + i EBI sums to
+  e^(-mua L) e^(-i (phase0 + k L)), and the element fluence ER + i EI,
+  integrated over the slab, to e^(-i phase0) (1 - e^(-(mua + i k) L)) /
+  (mua + i k); in 2D DEBR + i DEBI equals EBR + i EBI at normal exit.
 * Conservation: mua = 0 with scattering and a refractive-index jump: no
   absorption, no roulette, so all launched energy leaves through the
   boundary, each photon with weight 1.
@@ -19,6 +25,7 @@ The same cases run on the oracle (CPU) and on the GPU path (FP64 and FP32).
   between the two parallel faces, each reflecting with the unpolarised
   Fresnel reflectance R at that angle, so the bottom receives T / (1 - R^2)
   and the top R T / (1 - R^2).  Statistical (binomial), checked to 5 sigma.
+  In 2D, DEBR is EBR divided by the exit cosine.
 """
 import math
 
@@ -116,6 +123,31 @@ def check_beer_lambert(run, dim, tol):
     assert abs(absorbed(p, res) - (1.0 - t)) <= tol
 
 
+C0 = 2.99792458e11   # mm/s
+
+
+def check_fd_beer_lambert(run, dim, tol):
+    mua, thick, n = 0.1, 5, 1.3
+    p, faces = slab(dim, thick, 4, mua, 0.0, 0.9, n, n, nphoton=20000)
+    p.f, p.phase0 = 5e9, 0.3
+    res = run(p)
+    k = 2.0 * math.pi * p.f / C0 * n
+    area = _tri_areas(np.asarray(p.r), np.asarray(p.BH))
+    bot = faces["bottom"]
+    got = complex(np.sum((res.EBR * area)[bot]), np.sum((res.EBI * area)[bot]))
+    want = math.exp(-mua * thick) * complex(math.cos(p.phase0 + k * thick), -math.sin(p.phase0 + k * thick))
+    assert res.loss == 0
+    assert abs(got - want) <= tol, (got, want)
+    vol = _volumes(np.asarray(p.r), np.asarray(p.H))
+    flu = complex(np.sum(res.ER * vol), np.sum(res.EI * vol))
+    z = complex(mua, k)
+    want_flu = complex(math.cos(p.phase0), -math.sin(p.phase0)) * (1.0 - np.exp(-z * thick)) / z
+    assert abs(flu - want_flu) <= tol * abs(want_flu), (flu, want_flu)
+    if dim == 2:
+        d = complex(np.sum((res.DEBR * area)[bot]), np.sum((res.DEBI * area)[bot]))
+        assert abs(d - got) <= tol, (d, got)
+
+
 def check_conservation(run, dim, tol):
     p, faces = slab(dim, 4, 3, 0.0, 1.0, 0.8, 1.4, 1.0, src_half=1, nphoton=20000)
     res = run(p)
@@ -144,6 +176,11 @@ def check_fresnel(run, dim, nphoton, cos_i):
     assert abs(ex["top"] - want_top) < 5 * sigma, (ex, want_top, sigma)
     assert abs(ex["bottom"] - T / (1.0 - R * R)) < 5 * sigma, (ex, T / (1.0 - R * R))
     assert ex["side"] < 1e-6, ex
+    if dim == 2:
+        d = res.DEBR * _tri_areas(np.asarray(p.r), np.asarray(p.BH))
+        for k in ("top", "bottom"):
+            # FP32 directions carry ~1e-7 relative error in the cosine
+            assert abs(float(d[faces[k]].sum()) * cos_i - ex[k]) <= 1e-6, k
 
 
 def _oracle(p):
@@ -153,6 +190,11 @@ def _oracle(p):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_oracle_beer_lambert(dim):
     check_beer_lambert(_oracle, dim, 1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_frequency_domain_slab(dim):
+    check_fd_beer_lambert(_oracle, dim, 1e-12)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
@@ -177,7 +219,7 @@ def vmc():
 def _gpu(vmc, precision):
     def run(p):
         with vmc.Problem(p) as prob:
-            return prob.run(p.Nphoton, seed=p.seed, precision=precision)
+            return prob.run(p.Nphoton, seed=p.seed, precision=precision, f=p.f, phase0=p.phase0)
     return run
 
 
@@ -189,6 +231,13 @@ PREC = [("fp64", 1e-12), ("fp32", 2e-6)]
 @pytest.mark.parametrize("dim", [2, 3])
 def test_gpu_beer_lambert(vmc, dim, prec, tol):
     check_beer_lambert(_gpu(vmc, getattr(vmc, prec.upper())), dim, tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gpu_frequency_domain_slab(vmc, dim, prec, tol):
+    check_fd_beer_lambert(_gpu(vmc, getattr(vmc, prec.upper())), dim, tol)
 
 
 @pytest.mark.gpu
