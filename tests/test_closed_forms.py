@@ -17,6 +17,11 @@ The same cases run on the oracle (CPU) and on the GPU path (FP64 and FP32).
   e^(-mua L) e^(-i (phase0 + k L)), and the element fluence ER + i EI,
   integrated over the slab, to e^(-i phase0) (1 - e^(-(mua + i k) L)) /
   (mua + i k); in 2D DEBR + i DEBI equals EBR + i EBI at normal exit.
+* Henyey-Greenstein at g = +1 and g = -1 (3D): g = 1 scatters straight
+  ahead, so with scattering on the slab still transmits exactly
+  e^(-mua L); g = -1 reverses the direction at every event, which is the
+  rod model with backscattering rate mus: without absorption the slab
+  transmits 1 / (1 + mus L) and reflects the rest (statistical, 5 sigma).
 * Conservation: mua = 0 with scattering and a refractive-index jump: no
   absorption, no roulette, so all launched energy leaves through the
   boundary, each photon with weight 1.
@@ -148,6 +153,29 @@ def check_fd_beer_lambert(run, dim, tol):
         assert abs(d - got) <= tol, (d, got)
 
 
+def check_forward_scattering(run, tol):
+    mua, thick = 0.1, 5
+    p, faces = slab(3, thick, 4, mua, 1.0, 1.0, 1.3, 1.3, nphoton=20000)
+    res = run(p)
+    ex = exitance(p, res, faces)
+    assert res.loss == 0
+    assert abs(ex["bottom"] - math.exp(-mua * thick)) <= tol, ex
+    assert ex["top"] == 0.0 and ex["side"] == 0.0, ex
+
+
+def check_rod_backscatter(run, nphoton):
+    mus, thick = 0.4, 5
+    p, faces = slab(3, thick, 4, 0.0, mus, -1.0, 1.3, 1.3, nphoton=nphoton)
+    res = run(p)
+    ex = exitance(p, res, faces)
+    t = 1.0 / (1.0 + mus * thick)
+    sigma = math.sqrt(t * (1.0 - t) / nphoton)
+    assert res.loss == 0
+    assert abs(ex["bottom"] - t) < 5 * sigma, (ex, t, sigma)
+    assert abs(ex["top"] - (1.0 - t)) < 5 * sigma, (ex, 1.0 - t)
+    assert ex["side"] == 0.0, ex
+
+
 def check_conservation(run, dim, tol):
     p, faces = slab(dim, 4, 3, 0.0, 1.0, 0.8, 1.4, 1.0, src_half=1, nphoton=20000)
     res = run(p)
@@ -197,6 +225,14 @@ def test_oracle_frequency_domain_slab(dim):
     check_fd_beer_lambert(_oracle, dim, 1e-12)
 
 
+def test_oracle_forward_scattering():
+    check_forward_scattering(_oracle, 1e-12)
+
+
+def test_oracle_rod_backscatter():
+    check_rod_backscatter(_oracle, 100000)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_oracle_conserves_energy_without_absorption(dim):
     check_conservation(_oracle, dim, 1e-12)
@@ -238,6 +274,18 @@ def test_gpu_beer_lambert(vmc, dim, prec, tol):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_gpu_frequency_domain_slab(vmc, dim, prec, tol):
     check_fd_beer_lambert(_gpu(vmc, getattr(vmc, prec.upper())), dim, tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", PREC)
+def test_gpu_forward_scattering(vmc, prec, tol):
+    check_forward_scattering(_gpu(vmc, getattr(vmc, prec.upper())), tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_gpu_rod_backscatter(vmc, prec):
+    check_rod_backscatter(_gpu(vmc, getattr(vmc, prec.upper())), 2000000)
 
 
 @pytest.mark.gpu
