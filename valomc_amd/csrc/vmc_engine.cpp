@@ -432,7 +432,7 @@ using namespace vmc;
 
 extern "C" {
 
-int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsources) {
+int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsources) try {
   if (!d) return VMC_ERR_INVALID_ARGUMENT;
   vmc_problem m;
   const int st = host_prepare(d, &m);
@@ -440,9 +440,11 @@ int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsour
   if (HN_out) std::memcpy(HN_out, m.HN.data(), sizeof(int64_t) * m.HN.size());
   if (nsources) *nsources = (int64_t)m.src.size();
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
+int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) try {
   if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   vmc_problem *m = new vmc_problem();
@@ -472,15 +474,17 @@ int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) {
     m->dbnd.ensure(m->bnd.size());
     hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
   } catch (HipFail f) {
-    int st = fail_hip(f);
+    const int err = fail_hip(f);
     vmc_problem_destroy(m);
-    return st;
+    return err;
   }
   *out = m;
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *d, vmc_problem **out) {
+int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *d, vmc_problem **out) try {
   if (!base || !d || !out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   std::shared_ptr<MediumData> med = base->med;
@@ -545,10 +549,12 @@ int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *
   }
   *out = m;
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_problem_set_medium(vmc_problem *m, const double *mua, const double *mus, const double *g,
-                           const double *n) {
+                           const double *n) try {
   if (!m) return VMC_ERR_INVALID_ARGUMENT;
   if (m->med.use_count() > 1) {   // shared with a variant: copy on write
     auto own = std::make_shared<MediumData>();
@@ -564,16 +570,20 @@ int vmc_problem_set_medium(vmc_problem *m, const double *mua, const double *mus,
   if (n) m->med->n.assign(n, n + m->Ne);
   m->med->dirty32 = m->med->dirty64 = true;
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) {
+int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) try {
   if (!m || !HN) return VMC_ERR_INVALID_ARGUMENT;
   std::memcpy(HN, m->HN.data(), sizeof(int64_t) * m->HN.size());
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
-                    vmc_progress_fn progress, void *user) {
+                    vmc_progress_fn progress, void *user) try {
   if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
   const auto t_start = std::chrono::steady_clock::now();
   const bool fd = 2.0 * M_PI * run->f > 0.0;
@@ -626,10 +636,12 @@ int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vm
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
   return aborted ? VMC_ERR_ABORTED : VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *tuned, int64_t trial_photons,
-                         int32_t *trials) {
+                         int32_t *trials) try {
   if (!m || !run || !tuned || run->Nphoton < 0 || trial_photons < 0) return VMC_ERR_INVALID_ARGUMENT;
   {
     LaunchCfg c;
@@ -647,10 +659,12 @@ int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *
   });
   tuned->Nphoton = run->Nphoton;
   return st;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_fraction, vmc_run_desc *tuned,
-                       int32_t *trials) {
+                       int32_t *trials) try {
   if (!items || n <= 0 || !tuned || !(trial_fraction >= 0.0 && trial_fraction <= 1.0)) return VMC_ERR_INVALID_ARGUMENT;
   const double frac = trial_fraction > 0.0 ? trial_fraction : 0.02;
   // trial batches: the same items with a fraction of their photons and no
@@ -677,9 +691,11 @@ int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_frac
     vmc_stats s{};
     return vmc_batch_run(trial.data(), n, &s) == VMC_OK ? s.kernel_ms : std::numeric_limits<double>::infinity();
   });
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
+int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
   const auto t_start = std::chrono::steady_clock::now();
   std::vector<BatchGroup> groups;
   {
@@ -723,9 +739,11 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) {
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan **out) {
+int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan **out) try {
   if (!out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   std::vector<BatchGroup> groups;
@@ -782,9 +800,11 @@ int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan
   }
   *out = pl.release();
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
-int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc_stats *stats) {
+int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
   if (!pl || !items || n != (int32_t)pl->probs.size()) return VMC_ERR_INVALID_ARGUMENT;
   for (int32_t i = 0; i < n; ++i) {
     const vmc_run_desc &r = items[i].run;
@@ -837,6 +857,8 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 void vmc_batch_plan_destroy(vmc_batch_plan *pl) { delete pl; }
@@ -856,7 +878,7 @@ int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64
                     const double *mua, const double *mus, const double *g, const double *n, double f,
                     double phase0, int64_t Nphoton, const double *GaussianSigma, uint64_t seed,
                     int32_t precision, double *ER, double *EI, double *EBR, double *EBI, double *DEBR,
-                    double *DEBI, int64_t *HN_out, int64_t *loss) {
+                    double *DEBI, int64_t *HN_out, int64_t *loss) try {
   vmc_problem_desc d{};
   d.dim = dim;
   d.Ne = Ne; d.Nb = Nb; d.Nr = Nr;
@@ -883,6 +905,8 @@ int vmc_monte_carlo(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64
   if (loss) *loss = s.loss;
   vmc_problem_destroy(p);
   return st;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 const char *vmc_status_string(int st) {

@@ -14,6 +14,7 @@
 #include <climits>
 #include <cstdint>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -195,6 +196,15 @@ void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t
                double *EBR, double *EBI, double *DEBR, double *DEBI);
 // a HIP failure as a status, with its text in g_last_error
 int fail_hip(const HipFail &f);
+// a failed host allocation as a status (the C entry points catch
+// std::bad_alloc, so no exception leaves the library)
+inline int host_out_of_memory() noexcept {
+  try {
+    g_last_error = "out of host memory";
+  } catch (...) {
+  }
+  return VMC_ERR_OUT_OF_MEMORY;
+}
 template <class T> void copy_in(std::vector<T> &dst, const T *src, int64_t n) {
   if (src) dst.assign(src, src + n);
   else dst.clear();

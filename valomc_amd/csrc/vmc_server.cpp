@@ -154,6 +154,8 @@ void finisher_loop(vmc_server *s) {
         server_finish(s, r);
       } catch (HipFail f) {
         r->status = fail_hip(f);
+      } catch (const std::bad_alloc &) {
+        r->status = host_out_of_memory();
       }
     }
     std::lock_guard<std::mutex> lk(s->mu);
@@ -266,6 +268,14 @@ void server_loop(vmc_server *s) {
     tmpl.precision = s->desc.precision;
     tmpl.tally = s->desc.tally;
     std::vector<Request *> done_now;
+    // a HIP failure or a failed host allocation ends the launch's requests
+    auto fail_group = [&](int st) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (Request *r : group) {
+        r->status = st;
+        if (std::find(done_now.begin(), done_now.end(), r) == done_now.end()) done_now.push_back(r);
+      }
+    };
     try {
       Arena &a = s->arena;
       for (Request *r : admitted) {
@@ -337,12 +347,9 @@ void server_loop(vmc_server *s) {
             done_now.push_back(r);
       }
     } catch (HipFail f) {
-      const int st = fail_hip(f);
-      std::lock_guard<std::mutex> lk(s->mu);
-      for (Request *r : group) {
-        r->status = st;
-        if (std::find(done_now.begin(), done_now.end(), r) == done_now.end()) done_now.push_back(r);
-      }
+      fail_group(fail_hip(f));
+    } catch (const std::bad_alloc &) {
+      fail_group(host_out_of_memory());
     }
     // hand completed requests to the finisher; their last launch has
     // completed, so their tallies are final
@@ -359,7 +366,7 @@ void server_loop(vmc_server *s) {
 
 extern "C" {
 
-int vmc_server_create(const vmc_server_desc *d, vmc_server **out) {
+int vmc_server_create(const vmc_server_desc *d, vmc_server **out) try {
   if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
   if (d->precision != VMC_FP64 && d->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
@@ -414,10 +421,12 @@ int vmc_server_create(const vmc_server_desc *d, vmc_server **out) {
   raw->finisher = std::thread(finisher_loop, raw);
   *out = raw;
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_server_submit(vmc_server *s, vmc_problem *prob, const vmc_run_desc *run, const vmc_output *out,
-                      uint64_t *ticket) {
+                      uint64_t *ticket) try {
   if (!s || !prob || !run || !ticket || run->Nphoton < 0 || prob->device != s->device) return VMC_ERR_INVALID_ARGUMENT;
   std::unique_ptr<Request> r(new Request());
   r->prob = prob;
@@ -440,6 +449,8 @@ int vmc_server_submit(vmc_server *s, vmc_problem *prob, const vmc_run_desc *run,
   s->all[r->ticket] = std::move(r);
   s->cv.notify_all();
   return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_server_poll(vmc_server *s, uint64_t ticket, int32_t *done) {
@@ -451,7 +462,7 @@ int vmc_server_poll(vmc_server *s, uint64_t ticket, int32_t *done) {
   return VMC_OK;
 }
 
-int vmc_server_wait(vmc_server *s, uint64_t ticket, vmc_stats *stats) {
+int vmc_server_wait(vmc_server *s, uint64_t ticket, vmc_stats *stats) try {
   if (!s) return VMC_ERR_INVALID_ARGUMENT;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->all.find(ticket);
@@ -468,6 +479,8 @@ int vmc_server_wait(vmc_server *s, uint64_t ticket, vmc_stats *stats) {
   }
   s->all.erase(it);
   return st;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
 }
 
 int vmc_server_counters(vmc_server *s, int64_t *launches, int64_t *photons, double *busy_ms, int32_t *waiting,
