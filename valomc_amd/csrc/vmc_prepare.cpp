@@ -89,20 +89,20 @@ void error_checks(const vmc_problem &m) {
 // Neighbour table.  Same result as the reference's vertex-incidence search
 // (MC3D.hpp:740-871, MC2D.hpp:645-773): face f of element e gets the other
 // element, or boundary entry -1-ib, holding all of its vertices; two
-// different matches are an inconsistent mesh.  Found here by sorting face
-// keys, O(F log F) instead of O(sum of vertex degree^2).
+// different matches are an inconsistent mesh.  Found here by bucketing face
+// keys by their smallest vertex (a counting sort, O(F)) and sorting each
+// small bucket, instead of the reference's O(sum of vertex degree^2).
 // ---------------------------------------------------------------------------
 
 struct FaceKey {
-  int64_t a, b, c;
+  int64_t b, c;    // the face's other vertices, ascending (c = -1 for 2D edges)
   int64_t owner;   // (el << 2 | f) for element faces, -1 - ib for boundary entries
   bool operator<(const FaceKey &o) const {
-    if (a != o.a) return a < o.a;
     if (b != o.b) return b < o.b;
     if (c != o.c) return c < o.c;
     return owner < o.owner;
   }
-  bool same(const FaceKey &o) const { return a == o.a && b == o.b && c == o.c; }
+  bool same(const FaceKey &o) const { return b == o.b && c == o.c; }
 };
 
 inline void sort3(int64_t &a, int64_t &b, int64_t &c) {
@@ -114,57 +114,76 @@ inline void sort3(int64_t &a, int64_t &b, int64_t &c) {
 void build_neighbours(vmc_problem &m) {
   if ((int64_t)m.HN.size() == m.Ne * m.nv) return;   // given (MC3D.hpp:839)
   m.HN.assign(m.Ne * m.nv, kUnset);
-  std::vector<FaceKey> keys;
-  keys.reserve(m.Ne * m.nv + m.Nb);
-  for (int64_t e = 0; e < m.Ne; ++e)
-    for (int f = 0; f < m.nv; ++f) {
-      FaceKey k;
+  // key k of the element faces, then of the boundary entries: smallest
+  // vertex a, the others b <= c (vertex indices are checked against Nr)
+  auto key = [&](int64_t k, int64_t &a, FaceKey &fk) {
+    if (k < m.Ne * m.nv) {
+      const int64_t e = k / m.nv;
+      const int f = (int)(k % m.nv);
       if (m.dim == 3) {
-        k.a = m.h(e, kFace3[f][0]); k.b = m.h(e, kFace3[f][1]); k.c = m.h(e, kFace3[f][2]);
-        sort3(k.a, k.b, k.c);
+        a = m.h(e, kFace3[f][0]); fk.b = m.h(e, kFace3[f][1]); fk.c = m.h(e, kFace3[f][2]);
+        sort3(a, fk.b, fk.c);
       } else {
-        k.a = std::min(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
-        k.b = std::max(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
-        k.c = -1;
+        a = std::min(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        fk.b = std::max(m.h(e, kEdge2[f][0]), m.h(e, kEdge2[f][1]));
+        fk.c = -1;
       }
-      k.owner = (e << 2) | f;
-      keys.push_back(k);
-    }
-  for (int64_t b = 0; b < m.Nb; ++b) {
-    FaceKey k;
-    if (m.dim == 3) {
-      k.a = m.bh(b, 0); k.b = m.bh(b, 1); k.c = m.bh(b, 2);
-      sort3(k.a, k.b, k.c);
+      fk.owner = (e << 2) | f;
     } else {
-      k.a = std::min(m.bh(b, 0), m.bh(b, 1));
-      k.b = std::max(m.bh(b, 0), m.bh(b, 1));
-      k.c = -1;
-    }
-    k.owner = -1 - b;
-    keys.push_back(k);
-  }
-  std::sort(keys.begin(), keys.end());
-  const int dup = m.dim == 3 ? INCONSISTENT_MESH_DUPLICATE_NEIGHBORS : INCONSISTENT_MESH;
-  size_t i = 0;
-  while (i < keys.size()) {
-    size_t j = i + 1;
-    while (j < keys.size() && keys[j].same(keys[i])) ++j;
-    // every element face in the group takes the one other entry
-    for (size_t q = i; q < j; ++q) {
-      if (keys[q].owner < 0) continue;
-      const int64_t e = keys[q].owner >> 2;
-      const int f = (int)(keys[q].owner & 3);
-      int64_t val = kUnset;
-      for (size_t o = i; o < j; ++o) {
-        if (o == q) continue;
-        const int64_t cand = keys[o].owner < 0 ? keys[o].owner : (keys[o].owner >> 2);
-        if (cand == e) continue;
-        if (val != kUnset && val != cand) throw McError{dup};
-        val = cand;
+      const int64_t b = k - m.Ne * m.nv;
+      if (m.dim == 3) {
+        a = m.bh(b, 0); fk.b = m.bh(b, 1); fk.c = m.bh(b, 2);
+        sort3(a, fk.b, fk.c);
+      } else {
+        a = std::min(m.bh(b, 0), m.bh(b, 1));
+        fk.b = std::max(m.bh(b, 0), m.bh(b, 1));
+        fk.c = -1;
       }
-      m.HN[e + m.Ne * f] = val;
+      fk.owner = -1 - b;
     }
-    i = j;
+  };
+  const int64_t nk = m.Ne * m.nv + m.Nb;
+  std::vector<int64_t> start(m.Nr + 1, 0);
+  FaceKey fk;
+  int64_t a;
+  for (int64_t k = 0; k < nk; ++k) {
+    key(k, a, fk);
+    ++start[a + 1];
+  }
+  for (int64_t v = 0; v < m.Nr; ++v) start[v + 1] += start[v];
+  std::vector<FaceKey> keys(nk);
+  {
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t k = 0; k < nk; ++k) {
+      key(k, a, fk);
+      keys[fill[a]++] = fk;
+    }
+  }
+  const int dup = m.dim == 3 ? INCONSISTENT_MESH_DUPLICATE_NEIGHBORS : INCONSISTENT_MESH;
+  for (int64_t v = 0; v < m.Nr; ++v) {
+    const size_t lo = (size_t)start[v], hi = (size_t)start[v + 1];
+    std::sort(keys.begin() + lo, keys.begin() + hi);
+    size_t i = lo;
+    while (i < hi) {
+      size_t j = i + 1;
+      while (j < hi && keys[j].same(keys[i])) ++j;
+      // every element face in the group takes the one other entry
+      for (size_t q = i; q < j; ++q) {
+        if (keys[q].owner < 0) continue;
+        const int64_t e = keys[q].owner >> 2;
+        const int f = (int)(keys[q].owner & 3);
+        int64_t val = kUnset;
+        for (size_t o = i; o < j; ++o) {
+          if (o == q) continue;
+          const int64_t cand = keys[o].owner < 0 ? keys[o].owner : (keys[o].owner >> 2);
+          if (cand == e) continue;
+          if (val != kUnset && val != cand) throw McError{dup};
+          val = cand;
+        }
+        m.HN[e + m.Ne * f] = val;
+      }
+      i = j;
+    }
   }
   if (m.dim == 2)
     for (int64_t v : m.HN)
@@ -308,11 +327,12 @@ void build_order(vmc_problem &m) {
     key[e] = m.dim == 3 ? (spread_bits3(q[0]) | spread_bits3(q[1]) << 1 | spread_bits3(q[2]) << 2)
                         : (spread_bits2(q[0]) | spread_bits2(q[1]) << 1);
   }
+  // (key, element) pairs sort by key, ties by element
+  std::vector<std::pair<uint64_t, int32_t>> order(m.Ne);
+  for (int64_t e = 0; e < m.Ne; ++e) order[e] = {key[e], (int32_t)e};
+  std::sort(order.begin(), order.end());
   m.perm.resize(m.Ne);
-  for (int64_t e = 0; e < m.Ne; ++e) m.perm[e] = (int32_t)e;
-  std::sort(m.perm.begin(), m.perm.end(), [&](int32_t a, int32_t b) {
-    return key[a] != key[b] ? key[a] < key[b] : a < b;
-  });
+  for (int64_t d = 0; d < m.Ne; ++d) m.perm[d] = order[d].second;
   m.inv.resize(m.Ne);
   for (int64_t d = 0; d < m.Ne; ++d) m.inv[m.perm[d]] = (int32_t)d;
 }
