@@ -8,8 +8,9 @@
  * calls it.
  *
  * The same problem/result structs are used by the reference harness
- * (ref3d_driver.cpp / ref2d_driver.cpp), which runs the reference's own
- * engine compiled from /root/reference into oracle/_ref/.
+ * (ref_driver.cpp, built once per dimension), which runs the reference's
+ * own engine, compiled from the reference's sources by oracle/Makefile into
+ * oracle/_ref/.
  *
  * Array layout is the reference's: column-major (Fortran order), element
  * index fastest (cpp/3d/Array.hpp:58-66).  H[el + Ne*c], BH[ib + Nb*c],
