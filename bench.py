@@ -335,7 +335,9 @@ def main():
         desc = "384k-tet heterogeneous box, pencil source, CW, FP32 transport + FP64 tallies"
     p = make(nphot)
     fd = float(getattr(p, "f", 0.0) or 0.0) > 0.0
-    prob = vmc.Problem(p)
+    t_create = time.perf_counter()
+    prob = vmc.Problem(p)   # ErrorChecks + Init on the host, records to the device
+    t_create = time.perf_counter() - t_create
     kw = dict(block_size=a.block_size, blocks_per_cu=a.blocks_per_cu, refill_min=a.refill_min,
               replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64,
               f=float(getattr(p, "f", 0.0) or 0.0))
@@ -403,6 +405,7 @@ def main():
         "loss_per_step": runs[-1].loss,
         "rechecks_per_step": runs[-1].rechecks,
         "grid": [runs[-1].grid_blocks, runs[-1].block_size],
+        "problem_create_s": round(t_create, 3),
         **extra,
     }
     if tuned is not None:
