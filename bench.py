@@ -389,7 +389,7 @@ def main():
     balance = abs(led[0] + led[1] + led[2] - led[3] + led[4] - nphot) / nphot
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(make(1), a.cpu_seconds, desc.split(",")[0])
+        cpu = cpu_baseline(make(1), a.cpu_seconds, desc.split(", ")[0])
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
