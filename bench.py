@@ -262,9 +262,13 @@ def run_serve(a):
     with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
         _, wcal = serving.drive(srv, cache, pool, cal, lam0, seed=11)
         busy0 = srv.counters()["busy_ms"] / 1e3 / wcal
-    lam = lam0 * min(1.5, max(2.0 / 3.0, a.utilization / max(busy0, 1e-3)))
+    # the calibration requests' device time in the mix against their alone
+    # service times gives the sharing factor; the rate then targets the
+    # utilisation for the whole stream's alone service times
+    share = float(np.sum(service[:ncal])) / max(busy0 * wcal, 1e-9)
+    lam = lam0 * min(2.0, max(0.5, share))
     keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
-    print(f"serve: calibration busy {busy0:.3f}; {len(reqs)} requests at {lam:.2f}/s", file=sys.stderr,
+    print(f"serve: calibration busy {busy0:.3f}, sharing {share:.3f}; {len(reqs)} requests at {lam:.2f}/s", file=sys.stderr,
           flush=True)
     with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
         out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
@@ -289,7 +293,8 @@ def run_serve(a):
         "config": {"workload": "serving: Poisson stream, 40% 2D (1k-200k triangles) / 60% 3D (50k-2M tets) "
                                "resident jittered meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50",
                    "requests": len(out), "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
-                   "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3)},
+                   "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3),
+                                   "sharing_factor": round(share, 3)},
                    "photons_total": total, "policy": a.policy, "parallelism": "single GPU"},
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
