@@ -16,7 +16,8 @@
  *                undefined ('L', 'I' or 'p' with direction type 'n',
  *                MC3D.hpp:1112-1192 has no branch for them)
  *   18           invalid argument (NULL pointer, bad size or option)
- *   100 ..       device errors (see vmc_status_string / vmc_last_error)
+ *   100 ..       device errors (see vmc_status_string / vmc_last_error);
+ *                101 is also returned when a host allocation fails
  *
  * Threading: one problem handle per host thread; calls on one handle are
  * serialised by the caller.  Device memory and streams are owned by the

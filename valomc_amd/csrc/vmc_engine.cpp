@@ -936,7 +936,7 @@ const char *vmc_status_string(int st) {
     return "A light source of type L, I or p needs a light direction (type a, r or p)";
   case VMC_ERR_INVALID_ARGUMENT: return "Invalid argument";
   case VMC_ERR_HIP: return "HIP runtime error";
-  case VMC_ERR_OUT_OF_MEMORY: return "Out of device memory";
+  case VMC_ERR_OUT_OF_MEMORY: return "Out of memory (device, or host: see vmc_last_error)";
   case VMC_ERR_NO_DEVICE: return "No GPU device";
   case VMC_ERR_ABORTED: return "Computation aborted by the progress callback";
   default: return "Undefined error";
