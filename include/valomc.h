@@ -97,6 +97,12 @@ typedef struct vmc_run_desc {
                             slots again: 0 = default (1024 for batches of
                             several items, never for one), -1 = never       */
   int32_t reserved[1];
+  uint64_t first_photon; /* stream id of the run's first photon: the run walks
+                            photons first_photon .. first_photon + Nphoton - 1
+                            of the stream (seed, stream_tag), so disjoint
+                            ranges split one solve over launches or GPUs
+                            (raw sums add up exactly with fixed-point
+                            tallies); 0 = from the start                   */
 } vmc_run_desc;
 
 typedef struct vmc_output {

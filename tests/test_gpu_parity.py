@@ -115,6 +115,41 @@ def test_fixed_point_tallies_are_bit_reproducible(vmc):
     assert rel_l2(c.ER, a.ER) < 1e-8
 
 
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_iso_gauss_fd"])
+def test_photon_ranges_split_a_run(vmc, name):
+    """A run walks stream ids first_photon .. first_photon + N - 1, so
+    disjoint ranges (the shares of several launches or GPUs) add up to the
+    whole run: with fixed-point tallies the raw sums are exact integers, so
+    the photon-weighted outputs of the ranges equal the whole run's to the
+    rounding of the normalisation, and losses and element steps add up
+    exactly.  Ragged shares, one of a single photon."""
+    p = _golden(name)
+    n_all = 120000
+    cuts = [0, 17000, 60000, 60001, n_all]
+    print(f"case {name}", flush=True)
+    with vmc.Problem(p) as prob:
+        def run(n, first):
+            return prob.run(n, seed=21, f=p.f, phase0=p.phase0, precision=vmc.FP32, tally=vmc.TALLY_FIXED64,
+                            first_photon=first)
+        whole = run(n_all, 0)
+        parts = [(b - a, run(b - a, a)) for a, b in zip(cuts[:-1], cuts[1:])]
+    assert sum(r.loss for _, r in parts) == whole.loss
+    assert sum(r.steps for _, r in parts) == whole.steps
+    for k in OUT:
+        want = getattr(whole, k)
+        got = sum(getattr(r, k) * n for n, r in parts) / n_all
+        assert np.max(np.abs(got - want)) <= 1e-13 * max(np.max(np.abs(want)), 1e-300), k
+    # a range is the same photons whichever run they are in
+    with vmc.Problem(p) as prob:
+        a = prob.run(1000, seed=21, f=p.f, phase0=p.phase0, precision=vmc.FP32, tally=vmc.TALLY_FIXED64,
+                     first_photon=5000)
+        b = prob.run(1000, seed=21, f=p.f, phase0=p.phase0, precision=vmc.FP32, tally=vmc.TALLY_FIXED64,
+                     first_photon=5000, block_size=128)
+        c = prob.run(1000, seed=21, f=p.f, phase0=p.phase0, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+    assert all(np.array_equal(getattr(a, k), getattr(b, k)) for k in OUT)
+    assert not np.array_equal(a.ER, c.ER)
+
+
 @pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_iso_gauss_fd",
                                   "p2_mirror_hetero"])
 def test_tally_paths_agree_bit_for_bit(vmc, name):

@@ -57,6 +57,7 @@ class RunDesc(C.Structure):
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
         ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("direct_flush", C.c_int32),
         ("chunk", C.c_int32), ("cache_epoch", C.c_int32), ("reserved", C.c_int32 * 1),
+        ("first_photon", C.c_uint64),
     ]
 
 
@@ -177,7 +178,7 @@ def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float 
                   precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
                   blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
                   lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0,
-                  cache_epoch: int = 0) -> RunDesc:
+                  cache_epoch: int = 0, first_photon: int = 0) -> RunDesc:
     """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
     rd = RunDesc()
     rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
@@ -185,6 +186,7 @@ def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float 
     rd.block_size, rd.blocks_per_cu, rd.replicas = int(block_size), int(blocks_per_cu), int(replicas)
     rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
     rd.direct_flush, rd.chunk, rd.cache_epoch = int(direct_flush), int(chunk), int(cache_epoch)
+    rd.first_photon = int(first_photon)
     return rd
 
 
@@ -300,11 +302,13 @@ class Problem:
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
             lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0, cache_epoch: int = 0,
+            first_photon: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
-                           direct_flush=direct_flush, chunk=chunk, cache_epoch=cache_epoch)
+                           direct_flush=direct_flush, chunk=chunk, cache_epoch=cache_epoch,
+                           first_photon=first_photon)
         out, o = self._outputs()
         s = Stats()
         cb = None

@@ -218,7 +218,7 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
                   int64_t tbase, int64_t bbase) {
   ItemRec it{};
   it.count = count;
-  it.id0 = id0;
+  it.id0 = run->first_photon + id0;
   it.elems = f64 ? (const void *)m.med->elems64.p : (const void *)m.med->elems32.p;
   it.cdf = m.cdf.p;
   it.src = m.dsrc.p;
