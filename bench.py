@@ -34,6 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # optical parameters of the element, plus the tally read-modify-write
 B_STEP = {("fp32", False): 112, ("fp32", True): 128, ("fp64", False): 192, ("fp64", True): 208}
 B_STEP_2D = 96   # 2D FP32 CW: 3 edge lines + neighbour codes + optics + tally
+# vector arithmetic per element step, an estimate: 3-4 plane tests ~36 FLOP,
+# move 6, deposit and weight ~6, scatter amortised ~6, one or two
+# transcendentals ~6
+FLOP_STEP = 60
+FP32_PEAK_TFLOPS = 157.3
 
 
 def parse():
@@ -375,6 +380,10 @@ def main():
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
 
+    vector = {"achieved": round(steps_total * FLOP_STEP / kernel_s / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+              "unit": "TFLOP/s", "flop_per_step": FLOP_STEP}
+    vector["frac"] = round(vector["achieved"] / FP32_PEAK_TFLOPS, 4)
+
     extra = {}
     if not a.no_fp64:
         f32 = runs[0]
@@ -404,6 +413,7 @@ def main():
         "config": {"workload": desc, "elements": p.Ne, "boundary_faces": p.Nb, "nodes": p.Nr,
                    "photons_per_step": nphot, "parallelism": "single GPU"},
         "roofline": roofline,
+        "vector_roofline": vector,
         "cpu_baseline": cpu,
         "element_steps_per_s": round(steps_total / kernel_s, 1),
         "energy_balance_rel": balance,
