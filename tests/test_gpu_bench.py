@@ -38,6 +38,9 @@ def test_bench_line_contract():
                                           / 1e9, rel=2e-3)
     assert r["frac"] == pytest.approx(r["achieved"] / r["peak"], rel=2e-3)
     assert r["traffic"] is None or r["traffic"] > 0
+    v = d["vector_roofline"]
+    assert v["unit"] == "TFLOP/s" and v["peak"] == 157.3
+    assert v["frac"] == pytest.approx(v["achieved"] / v["peak"], rel=1e-2)
     c = d["cpu_baseline"]
     assert c["value"] > 0 and c["unit"] == "photons/s" and c["cores"] >= 1
     assert c["kind"] in ("reference", "port") and c["sample"]
