@@ -9,8 +9,36 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 namespace vmc {
+
+// Runs fn(lo, hi) over [0, n) split into contiguous ranges on up to 16
+// host threads (one range when n is small); the first exception a range
+// throws is rethrown here after all ranges end.
+template <class Fn>
+void parallel_ranges(int64_t n, Fn fn) {
+  const int64_t hw = (int64_t)std::max(1u, std::thread::hardware_concurrency());
+  const int64_t nt = std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, n / 65536));
+  if (nt <= 1) {
+    fn((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  std::vector<std::exception_ptr> err((size_t)nt);
+  for (int64_t t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      try {
+        fn(n * t / nt, n * (t + 1) / nt);
+      } catch (...) {
+        err[(size_t)t] = std::current_exception();
+      }
+    });
+  for (auto &th : pool) th.join();
+  for (auto &e : err)
+    if (e) std::rethrow_exception(e);
+}
 
 thread_local std::string g_last_error;
 
@@ -142,27 +170,38 @@ void build_neighbours(vmc_problem &m) {
       fk.owner = -1 - b;
     }
   };
+  // counting sort by a, on several threads: the order inside a bucket
+  // depends on the threads, but each bucket is fully sorted below
   const int64_t nk = m.Ne * m.nv + m.Nb;
   std::vector<int64_t> start(m.Nr + 1, 0);
-  FaceKey fk;
-  int64_t a;
-  for (int64_t k = 0; k < nk; ++k) {
-    key(k, a, fk);
-    ++start[a + 1];
-  }
+  parallel_ranges(nk, [&](int64_t k0, int64_t k1) {
+    FaceKey fk;
+    int64_t a;
+    for (int64_t k = k0; k < k1; ++k) {
+      key(k, a, fk);
+      __atomic_fetch_add(&start[a + 1], 1, __ATOMIC_RELAXED);
+    }
+  });
   for (int64_t v = 0; v < m.Nr; ++v) start[v + 1] += start[v];
-  std::vector<FaceKey> keys(nk);
+  std::unique_ptr<FaceKey[]> keys(new FaceKey[nk]);   // every entry written below
   {
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    for (int64_t k = 0; k < nk; ++k) {
-      key(k, a, fk);
-      keys[fill[a]++] = fk;
-    }
+    parallel_ranges(nk, [&](int64_t k0, int64_t k1) {
+      FaceKey fk;
+      int64_t a;
+      for (int64_t k = k0; k < k1; ++k) {
+        key(k, a, fk);
+        keys[__atomic_fetch_add(&fill[a], 1, __ATOMIC_RELAXED)] = fk;
+      }
+    });
   }
   const int dup = m.dim == 3 ? INCONSISTENT_MESH_DUPLICATE_NEIGHBORS : INCONSISTENT_MESH;
-  for (int64_t v = 0; v < m.Nr; ++v) {
+  // buckets are independent: ranges of vertices on separate threads, each
+  // face written by the one bucket that holds its key
+  parallel_ranges(m.Nr, [&](int64_t v0, int64_t v1) {
+  for (int64_t v = v0; v < v1; ++v) {
     const size_t lo = (size_t)start[v], hi = (size_t)start[v + 1];
-    std::sort(keys.begin() + lo, keys.begin() + hi);
+    std::sort(keys.get() + lo, keys.get() + hi);
     size_t i = lo;
     while (i < hi) {
       size_t j = i + 1;
@@ -185,6 +224,7 @@ void build_neighbours(vmc_problem &m) {
       i = j;
     }
   }
+  });
   if (m.dim == 2)
     for (int64_t v : m.HN)
       if (v == kUnset) throw McError{MISSING_BOUNDARY};   // MC2D.hpp:768-771
@@ -316,7 +356,8 @@ void build_order(vmc_problem &m) {
   const double inv_cell = (cell > 0 && maxspan / cell < maxcells) ? 1.0 / cell
                                                                   : (maxspan > 0 ? maxcells / maxspan : 1.0);
   std::vector<uint64_t> key(m.Ne);
-  for (int64_t e = 0; e < m.Ne; ++e) {
+  parallel_ranges(m.Ne, [&](int64_t e0, int64_t e1) {
+  for (int64_t e = e0; e < e1; ++e) {
     uint64_t q[3] = {0, 0, 0};
     for (int d = 0; d < m.dim; ++d) {
       double c = 0;
@@ -327,10 +368,34 @@ void build_order(vmc_problem &m) {
     key[e] = m.dim == 3 ? (spread_bits3(q[0]) | spread_bits3(q[1]) << 1 | spread_bits3(q[2]) << 2)
                         : (spread_bits2(q[0]) | spread_bits2(q[1]) << 1);
   }
-  // (key, element) pairs sort by key, ties by element
+  });
+  // (key, element) pairs sort by key, ties by element: ranges sorted on
+  // separate threads, then merged pairwise (the pairs are distinct, so the
+  // order is the one a single sort gives)
   std::vector<std::pair<uint64_t, int32_t>> order(m.Ne);
   for (int64_t e = 0; e < m.Ne; ++e) order[e] = {key[e], (int32_t)e};
-  std::sort(order.begin(), order.end());
+  std::vector<int64_t> bounds{0};
+  parallel_ranges(m.Ne, [&](int64_t e0, int64_t e1) { std::sort(order.begin() + e0, order.begin() + e1); });
+  {
+    // the same split parallel_ranges used, recomputed
+    const int64_t hw = (int64_t)std::max(1u, std::thread::hardware_concurrency());
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, m.Ne / 65536));
+    for (int64_t t = 1; t <= nt; ++t) bounds.push_back(m.Ne * t / nt);
+  }
+  while (bounds.size() > 2) {
+    std::vector<int64_t> next{0};
+    std::vector<std::thread> pool;
+    for (size_t k = 0; k + 2 < bounds.size(); k += 2) {
+      const int64_t b0 = bounds[k], b1 = bounds[k + 1], b2 = bounds[k + 2];
+      pool.emplace_back([&order, b0, b1, b2] {
+        std::inplace_merge(order.begin() + b0, order.begin() + b1, order.begin() + b2);
+      });
+      next.push_back(b2);
+    }
+    if (bounds.size() % 2 == 0) next.push_back(bounds.back());   // an odd range left over
+    for (auto &th : pool) th.join();
+    bounds.swap(next);
+  }
   m.perm.resize(m.Ne);
   for (int64_t d = 0; d < m.Ne; ++d) m.perm[d] = order[d].second;
   m.inv.resize(m.Ne);
@@ -434,7 +499,8 @@ void build_source_records(vmc_problem &m) {
 template <int NF, class R>
 void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
   out.assign(m.Ne, Elem<NF, R>{});
-  for (int64_t dev = 0; dev < m.Ne; ++dev) {
+  parallel_ranges(m.Ne, [&](int64_t d0, int64_t d1) {
+  for (int64_t dev = d0; dev < d1; ++dev) {
     const int64_t e = m.perm[dev];
     double v[4][3] = {{0}};
     for (int c = 0; c < m.nv; ++c)
@@ -482,6 +548,7 @@ void pack_elements(const vmc_problem &m, std::vector<Elem<NF, R>> &out) {
       E.nbo[f].y = nx >= 0 ? (R)m.med->n[nx] : R(0);
     }
   }
+  });
 }
 
 void build_nb_codes(vmc_problem &m) {
