@@ -10,31 +10,42 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <system_error>
 #include <thread>
 
 namespace vmc {
 
-// Runs fn(lo, hi) over [0, n) split into contiguous ranges on up to 16
-// host threads (one range when n is small); the first exception a range
-// throws is rethrown here after all ranges end.
+// Ranges [0, n) is split into by parallel_ranges: up to 16 host threads,
+// at least 65536 items each.
+inline int64_t range_count(int64_t n) {
+  const int64_t hw = (int64_t)std::max(1u, std::thread::hardware_concurrency());
+  return std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, n / 65536));
+}
+
+// Runs fn(lo, hi) over the range_count(n) contiguous ranges of [0, n), each
+// on a host thread (ranges whose thread cannot be started run on the
+// caller's); the first exception a range throws is rethrown here after all
+// ranges end.
 template <class Fn>
 void parallel_ranges(int64_t n, Fn fn) {
-  const int64_t hw = (int64_t)std::max(1u, std::thread::hardware_concurrency());
-  const int64_t nt = std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, n / 65536));
-  if (nt <= 1) {
-    fn((int64_t)0, n);
-    return;
-  }
-  std::vector<std::thread> pool;
+  const int64_t nt = range_count(n);
   std::vector<std::exception_ptr> err((size_t)nt);
-  for (int64_t t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      try {
-        fn(n * t / nt, n * (t + 1) / nt);
-      } catch (...) {
-        err[(size_t)t] = std::current_exception();
-      }
-    });
+  auto range = [&](int64_t t) {
+    try {
+      fn(n * t / nt, n * (t + 1) / nt);
+    } catch (...) {
+      err[(size_t)t] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int64_t t = 1; t < nt; ++t) {
+    try {
+      pool.emplace_back(range, t);
+    } catch (const std::system_error &) {
+      range(t);
+    }
+  }
+  range(0);
   for (auto &th : pool) th.join();
   for (auto &e : err)
     if (e) std::rethrow_exception(e);
@@ -374,22 +385,23 @@ void build_order(vmc_problem &m) {
   // order is the one a single sort gives)
   std::vector<std::pair<uint64_t, int32_t>> order(m.Ne);
   for (int64_t e = 0; e < m.Ne; ++e) order[e] = {key[e], (int32_t)e};
-  std::vector<int64_t> bounds{0};
   parallel_ranges(m.Ne, [&](int64_t e0, int64_t e1) { std::sort(order.begin() + e0, order.begin() + e1); });
-  {
-    // the same split parallel_ranges used, recomputed
-    const int64_t hw = (int64_t)std::max(1u, std::thread::hardware_concurrency());
-    const int64_t nt = std::min<int64_t>(std::min<int64_t>(hw, 16), std::max<int64_t>(1, m.Ne / 65536));
-    for (int64_t t = 1; t <= nt; ++t) bounds.push_back(m.Ne * t / nt);
-  }
+  std::vector<int64_t> bounds{0};   // the ranges parallel_ranges sorted
+  const int64_t nt = range_count(m.Ne);
+  for (int64_t t = 1; t <= nt; ++t) bounds.push_back(m.Ne * t / nt);
   while (bounds.size() > 2) {
     std::vector<int64_t> next{0};
     std::vector<std::thread> pool;
     for (size_t k = 0; k + 2 < bounds.size(); k += 2) {
       const int64_t b0 = bounds[k], b1 = bounds[k + 1], b2 = bounds[k + 2];
-      pool.emplace_back([&order, b0, b1, b2] {
+      auto merge = [&order, b0, b1, b2] {
         std::inplace_merge(order.begin() + b0, order.begin() + b1, order.begin() + b2);
-      });
+      };
+      try {
+        pool.emplace_back(merge);
+      } catch (const std::system_error &) {
+        merge();
+      }
       next.push_back(b2);
     }
     if (bounds.size() % 2 == 0) next.push_back(bounds.back());   // an odd range left over
