@@ -304,6 +304,17 @@ class Problem:
             lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0, cache_epoch: int = 0,
             first_photon: int = 0,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
+        """vmc_problem_run, MonteCarlo() of the reference (MC3D.hpp:1641-1822):
+        transport ``nphoton`` photons (stream ids first_photon ..
+        first_photon + nphoton - 1 of (seed, stream_tag)) and return the
+        normalised ER/EI/EBR/EBI/DEBR/DEBI in the caller's element order with
+        the run's statistics.  ``precision`` FP64 or FP32 (FP32 transport,
+        FP64 tallies); ``tally`` TALLY_FP64 or TALLY_FIXED64 (bit-reproducible);
+        the remaining keywords are the launch knobs of vmc_run_desc (0 =
+        default), which never change fixed-point results.  ``progress(percent)``
+        returning False aborts (status STATUS_ABORTED, outputs normalised by
+        the photons planned, as the reference does); ``check`` raises VmcError
+        on any other non-zero status."""
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
