@@ -141,6 +141,7 @@ def _status_string(st: int) -> str:
 
 
 def device_info() -> str:
+    """vmc_device_info: the GPU's name, CU count and memory."""
     lib = load_library()
     buf = C.create_string_buffer(256)
     ncu = C.c_int32(0)
@@ -156,6 +157,8 @@ def _colmajor(a, dtype):
 
 @dataclass
 class RunResult:
+    """Outputs of one solve (normalised as the reference leaves them, in the
+    caller's element order) and its statistics (vmc_stats)."""
     ER: np.ndarray
     EI: np.ndarray
     EBR: np.ndarray
@@ -285,6 +288,7 @@ class Problem:
         return out, o
 
     def hn(self) -> np.ndarray:
+        """vmc_problem_get_hn: the neighbour table Init() built, [Ne, dim+1]."""
         nv = self.dim + 1
         out = np.zeros(self.Ne * nv, dtype=np.int64)
         st = self.lib.vmc_problem_get_hn(self._h, out.ctypes.data)
@@ -293,6 +297,8 @@ class Problem:
         return out.reshape(nv, self.Ne).T.copy()
 
     def set_medium(self, mua=None, mus=None, g=None, n=None):
+        """vmc_problem_set_medium: new optical parameters per element (None
+        keeps the current ones) on the resident mesh."""
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mua, mus, g, n)]
         st = self.lib.vmc_problem_set_medium(self._h, *[None if a is None else a.ctypes.data for a in arrs])
         if st:
@@ -423,6 +429,9 @@ class BatchPlan:
         self._h = h
 
     def run(self, items=None):
+        """vmc_batch_plan_run: replay the recorded batch, with the items'
+        current seeds and media (or new (problem, run keywords) items of the
+        same shapes); returns (results, stats)."""
         items = self._items if items is None else list(items)
         arr, outs = _batch_array(items, self._common)
         s = Stats()
@@ -474,6 +483,8 @@ class Server:
         self._pending = {}
 
     def submit(self, prob: "Problem", nphoton: int, **kw) -> int:
+        """vmc_server_submit: queue a solve (Problem.run's keywords); returns
+        its ticket."""
         rd = make_run_desc(nphoton, **kw)
         out, o = prob._outputs()
         t = C.c_uint64()
@@ -484,6 +495,7 @@ class Server:
         return t.value
 
     def done(self, ticket: int) -> bool:
+        """vmc_server_poll: whether the ticket's outputs are written."""
         d = C.c_int32()
         st = self.lib.vmc_server_poll(self._h, C.c_uint64(ticket), C.byref(d))
         if st:
@@ -491,6 +503,7 @@ class Server:
         return bool(d.value)
 
     def wait(self, ticket: int) -> RunResult:
+        """vmc_server_wait: block until the ticket is done; its RunResult."""
         s = Stats()
         st = self.lib.vmc_server_wait(self._h, C.c_uint64(ticket), C.byref(s))
         out, _, n = self._pending.pop(ticket)
@@ -501,6 +514,7 @@ class Server:
                          block_size=0, **out)
 
     def counters(self):
+        """vmc_server_counters: launches, photons, busy_ms, waiting, active."""
         l, p, b = C.c_int64(), C.c_int64(), C.c_double()
         w, a = C.c_int32(), C.c_int32()
         self.lib.vmc_server_counters(self._h, C.byref(l), C.byref(p), C.byref(b), C.byref(w), C.byref(a))
@@ -567,12 +581,15 @@ class _EngineBase:
         return None
 
     def Init(self):
+        """ErrorChecks() + Init(): the members become a resident problem."""
         if self._problem is not None:
             self._problem.close()
         self._problem = Problem(self._as_problem())
         self.HN = self._problem.hn()
 
     def MonteCarlo(self, progress: Optional[Callable[[float], bool]] = None, finalchecks=None):
+        """MonteCarlo(progress, finalchecks): fills ER/EI/EBR/EBI/DEBR/DEBI and
+        loss as the reference's members."""
         if self._problem is None:
             self.Init()
         res = self._problem.run(self.Nphoton, seed=self.seed, f=self.f, phase0=self.phase0,
@@ -590,8 +607,10 @@ class _EngineBase:
 
 
 class MC3D(_EngineBase):
+    """The reference's class MC3D (cpp/3d/MC3D.hpp:108-172) over the C ABI."""
     dim = 3
 
 
 class MC2D(_EngineBase):
+    """The reference's class MC2D (cpp/2d/MC2D.hpp:113-180) over the C ABI."""
     dim = 2
