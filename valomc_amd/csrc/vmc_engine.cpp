@@ -243,29 +243,7 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
 void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
                     vmc_output *out) {
   if (!out || (!out->ER && !out->EI && !out->EBR && !out->EBI && !out->DEBR && !out->DEBI)) return;
-  const double omega = 2.0 * M_PI * f;
-  const bool fd = omega > 0.0;
-  std::vector<double> ER(m.Ne), EI(m.Ne, 0.0), EBR(m.Nb), EBI(m.Nb), DEBR(m.Nb), DEBI(m.Nb), kn(m.Ne);
-  for (int64_t i = 0; i < m.Ne; ++i) {
-    const int64_t dv = m.inv[i];
-    ER[i] = raw_e[W * dv];
-    if (fd) EI[i] = raw_e[W * dv + 1];
-    kn[i] = omega / kC0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
-  }
-  for (int64_t i = 0; i < m.Nb; ++i) {
-    EBR[i] = raw_b[4 * i];
-    EBI[i] = raw_b[4 * i + 1];
-    DEBR[i] = raw_b[4 * i + 2];
-    DEBI[i] = raw_b[4 * i + 3];
-  }
-  normalise(m, omega, kn.data(), N, ER.data(), EI.data(), EBR.data(), EBI.data(), DEBR.data(), DEBI.data());
-  if (!out) return;
-  if (out->ER) std::memcpy(out->ER, ER.data(), 8 * m.Ne);
-  if (out->EI) std::memcpy(out->EI, EI.data(), 8 * m.Ne);
-  if (out->EBR) std::memcpy(out->EBR, EBR.data(), 8 * m.Nb);
-  if (out->EBI) std::memcpy(out->EBI, EBI.data(), 8 * m.Nb);
-  if (out->DEBR) std::memcpy(out->DEBR, DEBR.data(), 8 * m.Nb);
-  if (out->DEBI) std::memcpy(out->DEBI, DEBI.data(), 8 * m.Nb);
+  normalise_into(m, 2.0 * M_PI * f, N, W, raw_e, raw_b, out);
 }
 
 int64_t sector_round(int64_t n) { return (n + kSectorElems - 1) / kSectorElems * kSectorElems; }

@@ -191,9 +191,10 @@ void build_source_records(vmc_problem &m);
 void build_boundary(vmc_problem &m);
 // element records of the requested precision on the device (when stale)
 void upload_params(vmc_problem &m, bool f64);
-// normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703)
-void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t N, double *ER, double *EI,
-               double *EBR, double *EBI, double *DEBR, double *DEBI);
+// normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703),
+// from raw device-order words straight into the caller's outputs
+void normalise_into(const vmc_problem &m, double omega, int64_t N, int W, const double *raw_e,
+                    const double *raw_b, vmc_output *out);
 // a HIP failure as a status, with its text in g_last_error
 int fail_hip(const HipFail &f);
 // a failed host allocation as a status (the C entry points catch

@@ -626,32 +626,41 @@ void upload_params(vmc_problem &m, bool f64) {
   (f64 ? m.med->dirty64 : m.med->dirty32) = false;
 }
 
-// Normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703)
-void normalise(const vmc_problem &m, double omega, const double *kfac_n, int64_t N, double *ER, double *EI,
-               double *EBR, double *EBI, double *DEBR, double *DEBI) {
-  const bool d3 = m.dim == 3;
-  if (omega <= 0.0) {
-    for (int64_t i = 0; i < m.Ne; ++i) {
-      if (m.med->mua[i] > 0.0) ER[i] /= m.med->mua[i] * m.vol[i] * (double)N;
-      else ER[i] /= m.vol[i] * (double)N;
-    }
-    for (int64_t i = 0; i < m.Nb; ++i) EBR[i] /= (double)N * m.area[i];
-    if (!d3)
-      for (int64_t i = 0; i < m.Nb; ++i) DEBR[i] /= (double)N * m.area[i];
-  } else {
-    for (int64_t i = 0; i < m.Ne; ++i) {
-      const double a = ER[i], b = EI[i], k = kfac_n[i];
-      ER[i] = (b * k + a * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
-      EI[i] = -(a * k - b * m.med->mua[i]) / (pow(k, 2) + pow(m.med->mua[i], 2)) / (double)N / m.vol[i];
-    }
-    for (int64_t i = 0; i < m.Nb; ++i) {
-      EBR[i] /= (double)N * m.area[i];
-      EBI[i] /= (double)N * m.area[i];
-      if (!d3) {
-        DEBR[i] /= (double)N * m.area[i];
-        DEBI[i] /= (double)N * m.area[i];
+// Normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703),
+// fused with the gather from device element order: raw element words
+// (W per element, device order) and boundary words (EBR, EBI, DEBR, DEBI
+// per face) become the caller's outputs in one pass, with the reference's
+// expressions.  Words the reference leaves unnormalised (EBI in CW, DEBR
+// and DEBI in 3D) are copied as they are; NULL outputs are skipped.
+void normalise_into(const vmc_problem &m, double omega, int64_t N, int W, const double *raw_e,
+                    const double *raw_b, vmc_output *out) {
+  const bool d3 = m.dim == 3, fd = omega > 0.0;
+  if (out->ER || out->EI)
+    parallel_ranges(m.Ne, [&](int64_t i0, int64_t i1) {
+      for (int64_t i = i0; i < i1; ++i) {
+        const int64_t dv = m.inv[i];
+        const double a = raw_e[W * dv], mua = m.med->mua[i];
+        double er, ei = 0.0;
+        if (!fd) {
+          if (mua > 0.0) er = a / (mua * m.vol[i] * (double)N);
+          else er = a / (m.vol[i] * (double)N);
+        } else {
+          const double b = raw_e[W * dv + 1];
+          const double k = omega / kC0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+          er = (b * k + a * mua) / (pow(k, 2) + pow(mua, 2)) / (double)N / m.vol[i];
+          ei = -(a * k - b * mua) / (pow(k, 2) + pow(mua, 2)) / (double)N / m.vol[i];
+        }
+        if (out->ER) out->ER[i] = er;
+        if (out->EI) out->EI[i] = ei;
       }
-    }
+    });
+  for (int64_t i = 0; i < m.Nb; ++i) {
+    const double na = (double)N * m.area[i];
+    const double *w = raw_b + 4 * i;
+    if (out->EBR) out->EBR[i] = w[0] / na;
+    if (out->EBI) out->EBI[i] = fd ? w[1] / na : w[1];
+    if (out->DEBR) out->DEBR[i] = d3 ? w[2] : w[2] / na;
+    if (out->DEBI) out->DEBI[i] = (d3 || !fd) ? w[3] : w[3] / na;
   }
 }
 
