@@ -32,6 +32,44 @@ def test_library_exports_every_declared_symbol():
     assert exported <= declared, exported - declared
 
 
+C_CALLER = r"""
+#include "valomc.h"
+#include <stdio.h>
+int main(void) {
+  vmc_problem_desc d = {0};
+  vmc_run_desc run = {0};
+  vmc_server_desc sd = {0};
+  vmc_output out = {0};
+  vmc_stats stats = {0};
+  run.precision = VMC_FP32; run.tally = VMC_TALLY_FIXED64; run.first_photon = 0;
+  sd.policy = VMC_SERVE_SHORTEST;
+  (void)d; (void)out; (void)stats;
+  printf("%s|%s|%d|%d|%d\n", vmc_version(), vmc_status_string(VMC_ERR_OUT_OF_MEMORY),
+         vmc_problem_create(NULL, NULL), run.tally + (int)run.first_photon, sd.policy);
+  return 0;
+}
+"""
+
+
+def test_header_serves_c_callers(tmp_path):
+    """include/valomc.h is plain C (the MEX gateway and FFI callers see C):
+    a C99 program that uses its types and constants compiles warning-free
+    and links to the library; its calls that need no GPU answer."""
+    vmc.load_library()
+    src, exe = tmp_path / "caller.c", tmp_path / "caller"
+    src.write_text(C_CALLER)
+    lib_dir = os.path.dirname(vmc.LIB_PATH)
+    cc = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                         "-L", lib_dir, "-lvalomc", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)],
+                        capture_output=True, text=True)
+    assert cc.returncode == 0, cc.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    version, oom, st, tally, policy = out.stdout.strip().split("|")
+    assert version.startswith("valomc") and "Out of memory" in oom and int(st) == 18
+    assert int(tally) == 1 and int(policy) == 0
+
+
 def test_status_strings():
     lib = vmc.load_library()
     assert lib.vmc_status_string(0) == b"Success"
