@@ -78,7 +78,8 @@ typedef struct vmc_run_desc {
   int32_t blocks_per_cu; /* resident workgroups per CU: 0 = as many as fit,
                             else at most this many                          */
   int32_t replicas;      /* tally copies: > 0 that many (device-scope atomics);
-                            0 = automatic: one per XCD (8, L2 atomics), or a
+                            0 = automatic: one per XCD (8, workgroup-scope
+                            atomics), or a
                             single one when a copy is 4..64 MiB             */
   int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 6 */
   uint32_t stream_tag;   /* extra stream word: runs with different tags are

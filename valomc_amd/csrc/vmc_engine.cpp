@@ -38,7 +38,7 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   // workgroups, two per CU
   c->block = run->block_size > 0 ? run->block_size : (c->f64 ? 256 : 1024);
   if (c->block % 64 != 0 || c->block > (c->f64 ? 256 : 1024)) return VMC_ERR_INVALID_ARGUMENT;
-  // replicas == 0: one per XCD, added with L2 atomics (choose_replicas may
+  // replicas == 0: one per XCD, workgroup-scope atomics (choose_replicas may
   // make it a single one for the problem's size); > 0: that many, picked by
   // workgroup index and added with device-scope atomics
   c->xcc = run->replicas <= 0;
@@ -75,13 +75,16 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   return VMC_OK;
 }
 
-// replicas == 0: the tally copies follow the size of one copy.  Up to an
-// XCD's L2 (4 MiB) one copy per XCD, added with L2 atomics, costs nothing
-// extra; a copy that exceeds the L2 but fits the Infinity Cache comfortably
-// (64 MiB) is better kept single, device-scope atomics meeting in the cache
-// instead of eight copies streaming through it (the 2M-tet FD box: +18%);
-// larger ones (many batch instances) go back to one per XCD, whose L2
-// atomics beat device-scope atomics that reach HBM.
+// replicas == 0: the tally copies follow the size of one copy.  The L2s
+// pass these FP64/u64 atomics on to the memory side either way (the TCC
+// counters count every atomic request as an EA atomic); what the copies
+// change is how many lines the requests spread over and whether they stay
+// in the Infinity Cache.  Up to an XCD's L2 (4 MiB) one copy per XCD,
+// added with workgroup-scope atomics, spreads them at no cost; a copy that
+// exceeds that but fits the Infinity Cache comfortably (64 MiB) is better
+// kept single, its device-scope atomics meeting in the cache instead of
+// eight copies streaming through it (the 2M-tet FD box: +18%); larger ones
+// (many batch instances) go back to one per XCD.
 void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb) {
   if (run->replicas > 0) return;
   const double bytes = ((double)ne * c.W + (double)nb * 4) * 8;
