@@ -15,10 +15,13 @@
 // Moeller-Trumbore test on gathered vertices (MC3D.hpp:1006-1057,
 // 1825-1869); for a photon inside a convex element both pick the same face.
 //
-// Tallies go to `nrep` replicas selected by blockIdx % nrep, so consecutive
-// workgroups (which the dispatcher spreads over the XCDs) add into
-// different copies and hot elements near a source are not serialised on a
-// single L2 line; the replicas are summed after the walk.
+// Tallies: the deposits of one element visit are summed in registers; a
+// workgroup's LDS sector cache (sectors of 8 consecutive elements) takes
+// the visits of the sectors that claimed its slots; other sectors are
+// summed in per-lane LDS buffers that leave as coalesced atomics when the
+// photon moves on.  Those go to one of `nrep` replicas, the XCD's own
+// (workgroup-scope atomics) or, for an explicit count, one picked by
+// workgroup index (device scope); the replicas are summed after the walk.
 #include "vmc_types.h"
 
 #include <type_traits>
