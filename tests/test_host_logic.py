@@ -156,3 +156,62 @@ def test_pencil_face_of_the_box():
     assert p.notes["pencil_face"] == 3282
     tri = np.asarray(p.r)[np.asarray(p.BH)[3282]]
     assert np.all(tri[:, 2] == 20.0)
+
+
+def _unjittered(p):
+    """The structured grid a jittered benchmark mesh was drawn from: the
+    same element table, nodes moved back to the lattice (interior nodes
+    move by at most 0.2 h, so rounding to the lattice recovers them)."""
+    r = np.asarray(p.r)
+    lo = r.min(axis=0)
+    h = p.notes["h"]
+    return lo + np.rint((r - lo) / h) * h, h
+
+
+def _check_jittered(p):
+    r0, h = _unjittered(p)
+    r = np.asarray(p.r)
+    q = meshgen.signed_measure(r, p.H) / meshgen.signed_measure(r0, p.H)
+    # every element keeps its orientation (and a minimum quality)
+    assert q.min() >= meshgen.JITTER_MIN_QUALITY, (p.notes, q.min(), int(np.sum(q <= 0)))
+    # the offsets stay within +-0.2 h, boundary nodes do not move
+    off = np.abs(r - r0)
+    assert off.max() <= 0.2 * h + 1e-12
+    lo, hi = r0.min(axis=0), r0.max(axis=0)
+    on_box = np.any((np.abs(r0 - lo) < 1e-9) | (np.abs(r0 - hi) < 1e-9), axis=1)
+    assert off[on_box].max() < 1e-9
+    assert off[~on_box].max() > 0.15 * h
+
+
+def test_jitter_redraws_offsets_that_invert_elements():
+    """An unconditional +-0.2h draw turns a few elements inside out (57 of
+    the config-3 box's tets); with the element table the offsets that would
+    are redrawn, and the draw stays seeded."""
+    r, H, BH = meshgen.grid_mesh(np.arange(12) + 0.5, np.arange(12) + 0.5, np.arange(12) + 0.5)
+    raw = meshgen.jitter_interior(r, 1.0, 0.45, seed=3)
+    assert np.sum(meshgen.signed_measure(raw, H) / meshgen.signed_measure(r, H) <= 0) > 0
+    fixed = meshgen.jitter_interior(r, 1.0, 0.45, seed=3, H=H)
+    q = meshgen.signed_measure(fixed, H) / meshgen.signed_measure(r, H)
+    assert q.min() >= meshgen.JITTER_MIN_QUALITY
+    assert np.array_equal(fixed, meshgen.jitter_interior(r, 1.0, 0.45, seed=3, H=H))
+    assert np.abs(fixed - r).max() <= 0.45
+
+
+def test_fd_box_and_serving_meshes_keep_their_orientation():
+    p = meshgen.fd_box_3d(nphoton=1, seed=7)
+    assert p.Ne == 2035200
+    _check_jittered(p)
+    for q in meshgen.serving_pool(seed=2024):
+        _check_jittered(q)
+
+
+def test_oracle_loses_no_photon_on_the_fd_box():
+    """On the repaired 2M-tet box no photon leaves the mesh topology (the
+    inverted tets of an unconditional jitter lost 279 of 20,000)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    p = meshgen.fd_box_3d(nphoton=20000, seed=7)
+    r = pyoracle.run(p, 8, pyoracle.RNG_MT19937)
+    assert r.loss == 0
+    led = r.ledger
+    assert abs(led.sum() - 2 * led[3] - p.Nphoton) <= 1e-9 * p.Nphoton
