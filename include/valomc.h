@@ -39,6 +39,7 @@ extern "C" {
 #define VMC_ERR_OUT_OF_MEMORY 101
 #define VMC_ERR_NO_DEVICE 102
 #define VMC_ERR_ABORTED 103
+#define VMC_ERR_TALLY_RANGE 104 /* VMC_TALLY_FIXED64: a deposit left the exact range */
 
 /* transport arithmetic */
 #define VMC_FP64 0 /* plain double-precision transport (the GPU FP64 oracle)   */
@@ -46,7 +47,17 @@ extern "C" {
 
 /* tally accumulation */
 #define VMC_TALLY_FP64 0    /* double-precision atomics (order-dependent rounding) */
-#define VMC_TALLY_FIXED64 1 /* 64-bit fixed point, 2^-32 resolution: bit-reproducible */
+#define VMC_TALLY_FIXED64 1 /* 64-bit fixed point, 2^-32 resolution: bit-reproducible.
+                               Each deposit is rounded to a multiple of 2^-32
+                               and summed in int64, so every tally word's sum
+                               over a run must stay below 2^31 in raw units
+                               (weight; weight x mm in non-absorbing
+                               elements; weight / exit cosine for 2D DEBR).
+                               Runs are limited to fewer than 2^31 photons
+                               (VMC_ERR_INVALID_ARGUMENT), and a single
+                               deposit at or beyond 2^30, or not finite,
+                               ends the run with VMC_ERR_TALLY_RANGE
+                               (outputs are written but not exact).        */
 
 /* Geometry, medium and boundary conditions: everything MC3D/MC2D holds
    before Init().  dim = 3 for tetrahedra (MC3D), 2 for triangles (MC2D). */
@@ -97,13 +108,26 @@ typedef struct vmc_run_desc {
                             sector cache to the replica and lets sectors claim
                             slots again: 0 = default (1024 for batches of
                             several items, never for one), -1 = never       */
-  int32_t reserved[1];
+  int32_t model;         /* 1: weight0, chance and c0 below are the caller's;
+                            0: the reference's defaults 0.001, 0.1 and
+                            2.99792458e11 (MC3D.hpp:178-183), whatever the
+                            fields hold                                      */
   uint64_t first_photon; /* stream id of the run's first photon: the run walks
                             photons first_photon .. first_photon + Nphoton - 1
                             of the stream (seed, stream_tag), so disjoint
                             ranges split one solve over launches or GPUs
                             (raw sums add up exactly with fixed-point
                             tallies); 0 = from the start                   */
+  /* Model constants, the reference's public members MC3D::weight0, chance
+     and c0 (MC3D.hpp:146,160; MC2D.hpp:153,167), read when model == 1.  A
+     packet whose weight is below weight0 at an interior face crossing plays
+     roulette and survives with probability chance, its weight divided by
+     chance (MC3D.hpp:1578-1604); c0 is the speed of light in mm/s, so the
+     wave number is k = 2 pi f n / c0 (MC3D.hpp:662).  weight0 >= 0,
+     0 < chance <= 1, c0 > 0, all finite.                                   */
+  double weight0;
+  double chance;
+  double c0;
 } vmc_run_desc;
 
 typedef struct vmc_output {

@@ -27,7 +27,13 @@ class _Problem(C.Structure):
         ("BCType", C.c_void_p), ("BCn", C.c_void_p), ("BCLNormal", C.c_void_p),
         ("BCLDirType", C.c_void_p), ("BCIntensity", C.c_void_p), ("GaussianSigma", C.c_void_p),
         ("f", C.c_double), ("phase0", C.c_double), ("Nphoton", C.c_int64), ("seed", C.c_uint64),
+        ("weight0", C.c_double), ("chance", C.c_double), ("c0", C.c_double),
+        ("stream_tag", C.c_uint32), ("pad", C.c_uint32), ("first_photon", C.c_uint64),
     ]
+
+
+# the reference's model constants (MC3D.hpp:178-183)
+WEIGHT0, CHANCE, C0 = 0.001, 0.1, 2.99792458e11
 
 
 class _Result(C.Structure):
@@ -79,7 +85,7 @@ def _colmajor(a, dtype):
     return np.asfortranarray(np.asarray(a, dtype=dtype)).ravel(order="F")
 
 
-def _marshal(p):
+def _marshal(p, stream_tag=0, first_photon=0):
     keep = []
 
     def ptr(a, dtype):
@@ -105,11 +111,15 @@ def _marshal(p):
     s.GaussianSigma = ptr(p.GaussianSigma, np.float64)
     s.f, s.phase0 = float(p.f), float(p.phase0)
     s.Nphoton, s.seed = int(p.Nphoton), int(p.seed) & 0xFFFFFFFFFFFFFFFF
+    s.weight0 = float(getattr(p, "weight0", WEIGHT0))
+    s.chance = float(getattr(p, "chance", CHANCE))
+    s.c0 = float(getattr(p, "c0", C0))
+    s.stream_tag, s.first_photon = int(stream_tag), int(first_photon)
     return s, keep
 
 
-def _call(fn, p, *args) -> Result:
-    s, keep = _marshal(p)
+def _call(fn, p, *args, stream_tag=0, first_photon=0) -> Result:
+    s, keep = _marshal(p, stream_tag, first_photon)
     nv = p.dim + 1
     out = dict(ER=np.zeros(p.Ne), EI=np.zeros(p.Ne), EBR=np.zeros(p.Nb), EBI=np.zeros(p.Nb),
                DEBR=np.zeros(p.Nb), DEBI=np.zeros(p.Nb), HN=np.zeros(p.Ne * nv, dtype=np.int64))
@@ -124,12 +134,17 @@ def _call(fn, p, *args) -> Result:
                   ledger=np.array(list(res.ledger)), seconds=float(res.seconds))
 
 
-def run(p, nthreads: int = 1, rng_mode: int = RNG_MT19937) -> Result:
-    """Run the CPU restatement."""
+def run(p, nthreads: int = 1, rng_mode: int = RNG_MT19937, stream_tag: int = 0,
+        first_photon: int = 0) -> Result:
+    """Run the CPU restatement.  ``p`` may carry the model constants
+    ``weight0``, ``chance`` and ``c0`` (the reference's defaults otherwise);
+    in Philox mode photon i of the run draws from the stream (p.seed,
+    stream_tag, first_photon + i), as the GPU's run with that tag and first
+    photon does."""
     lib = _load("liboracle.so")
     fn = lib.oracle_run3d if p.dim == 3 else lib.oracle_run2d
     fn.restype = C.c_int
-    return _call(fn, p, C.c_int(nthreads), C.c_int(rng_mode))
+    return _call(fn, p, C.c_int(nthreads), C.c_int(rng_mode), stream_tag=stream_tag, first_photon=first_photon)
 
 
 def run_reference(p, nthreads: int = 1) -> Result:

@@ -69,6 +69,9 @@ extern "C" int REF_ENTRY(const oracle_problem *p, int nthreads, oracle_result *o
   if (MC.phase0 < 0) MC.phase0 += 2 * M_PI * ceil(-MC.phase0 / (2 * M_PI));
   MC.Nphoton = p->Nphoton;
   MC.seed = (unsigned long)p->seed;
+  MC.weight0 = p->weight0;
+  MC.chance = p->chance;
+  MC.c0 = p->c0;
   try {
     MC.ErrorChecks();
     MC.Init();

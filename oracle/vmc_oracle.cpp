@@ -151,8 +151,8 @@ struct Model {
   std::vector<char> BCType, BCLDirType;
   std::vector<double> BCn, BCLNormal, BCIntensity, GaussianSigma;
   double f = 0, omega = 0, phase0 = 0;
-  const double c0 = 2.99792458e11;           /* MC3D.hpp:178 */
-  const double weight0 = 0.001, chance = 0.1; /* MC3D.hpp:182-183 */
+  double c0 = 2.99792458e11;           /* MC3D.hpp:178, the caller's member */
+  double weight0 = 0.001, chance = 0.1; /* MC3D.hpp:182-183 */
   std::vector<int64_t> lsFace, lsMother;
   std::vector<double> lsCDF;
 
@@ -184,6 +184,9 @@ void load_model(Model &m, const oracle_problem *p, int dim) {
   if (p->GaussianSigma) m.GaussianSigma.assign(p->GaussianSigma, p->GaussianSigma + p->Nb);
   m.f = p->f;
   m.phase0 = p->phase0;
+  m.weight0 = p->weight0;
+  m.chance = p->chance;
+  m.c0 = p->c0;
 }
 
 /* Input checks in the order of MC3D::ErrorChecks (MC3D.hpp:460-571) and
@@ -992,8 +995,8 @@ int run(const oracle_problem *p, int nthreads, int rng_mode, oracle_result *out,
       for (int q = 0; q < 5; ++q) tled[t][q] = w.led[q];
     };
     if (rng_mode == ORACLE_RNG_PHILOX) {
-      Walker<PhiloxStream> w(m, PhiloxStream(p->seed));
-      for (int64_t i = 0; i < count[t]; ++i) w.run_photon((uint64_t)(first[t] + i));
+      Walker<PhiloxStream> w(m, PhiloxStream(p->seed, p->stream_tag));
+      for (int64_t i = 0; i < count[t]; ++i) w.run_photon(p->first_photon + (uint64_t)(first[t] + i));
       finish(w);
     } else {
       /* seed*totalthreads + t with totalthreads == 1 (MC3D.hpp:1658) */

@@ -41,6 +41,14 @@ typedef struct oracle_problem {
   double f, phase0;
   int64_t Nphoton;
   uint64_t seed;
+  /* model constants, the reference's public members MC3D::weight0, chance,
+     c0 (MC3D.hpp:146,160; defaults 0.001, 0.1, 2.99792458e11 at :178-183) */
+  double weight0, chance, c0;
+  /* Philox mode only: the stream tag and the stream id of the first photon
+     (photon i of the run draws from stream (seed, tag, first_photon + i)) */
+  uint32_t stream_tag;
+  uint32_t pad;
+  uint64_t first_photon;
 } oracle_problem;
 
 typedef struct oracle_result {

@@ -190,6 +190,28 @@ def p2_mirror_hetero():
     return p
 
 
+def p3_roulette_constants():
+    """Non-default model constants (MC3D::weight0, chance, c0): roulette
+    from weight 0.3 with survival chance 0.2, and a slower light speed in
+    the frequency domain."""
+    r, H, BH = _box(4)
+    p = mg.homogeneous(3, r, H, BH, 0.4, 2.0, 0.8, 1.4, 1.0)
+    top = _faces_on(r, BH, 2, 2.0)
+    p.BCType[top[:8]] = ord("l")
+    p.f, p.phase0 = 150e6, 0.4
+    p.weight0, p.chance, p.c0 = 0.3, 0.2, 2.2e11
+    p.Nphoton, p.seed = 20000, 23
+    return p
+
+
+def p2_roulette_constants():
+    """2D with a roulette that starts at weight 0.4 and revives half."""
+    p = mg.square_2d(nphoton=20000, seed=29)
+    p.mua[:] = 0.2
+    p.weight0, p.chance = 0.4, 0.5
+    return p
+
+
 CASES = {
     "p3_pencil_sphere": p3_pencil_sphere,
     "p3_cosinic_hetero_n": p3_cosinic_hetero_n,
@@ -204,6 +226,8 @@ CASES = {
     "p2_iso_gauss_fd": p2_iso_gauss_fd,
     "p2_given_directions": p2_given_directions,
     "p2_mirror_hetero": p2_mirror_hetero,
+    "p3_roulette_constants": p3_roulette_constants,
+    "p2_roulette_constants": p2_roulette_constants,
 }
 
 THREADS = (1, 4)
@@ -258,6 +282,9 @@ ERROR_CASES = {
 INPUT_KEYS = ("H", "BH", "r", "mua", "mus", "g", "n", "BCType", "BCn", "BCLNormal",
               "BCLDirType", "BCIntensity", "GaussianSigma", "HN")
 SCALAR_KEYS = ("dim", "f", "phase0", "Nphoton", "seed")
+# model constants, stored when a case sets them (the reference's defaults
+# otherwise, MC3D.hpp:178-183)
+MODEL_KEYS = {"weight0": 0.001, "chance": 0.1, "c0": 2.99792458e11}
 
 
 def to_arrays(p) -> dict:
@@ -268,6 +295,9 @@ def to_arrays(p) -> dict:
             d["in_" + k] = np.asarray(v)
     for k in SCALAR_KEYS:
         d["in_" + k] = np.asarray(getattr(p, k))
+    for k, default in MODEL_KEYS.items():
+        if float(getattr(p, k)) != default:
+            d["in_" + k] = np.asarray(float(getattr(p, k)))
     return d
 
 
@@ -281,4 +311,6 @@ def from_arrays(d) -> "mg.Problem":
         setattr(p, k, kw[k])
     p.f, p.phase0 = float(d["in_f"]), float(d["in_phase0"])
     p.Nphoton, p.seed = int(d["in_Nphoton"]), int(d["in_seed"])
+    for k, default in MODEL_KEYS.items():
+        setattr(p, k, float(d["in_" + k]) if ("in_" + k) in d else default)
     return p

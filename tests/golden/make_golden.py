@@ -7,7 +7,9 @@ normalised outputs (ER, EI, EBR, EBI, DEBR, DEBI, HN, loss) as
 <case>.npz.  Error cases store the status the reference reports
 (1 + mcerror).  Usage:
 
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [case ...]
+
+With case names, only those cases are written.
 """
 from __future__ import annotations
 
@@ -29,7 +31,10 @@ import pyoracle  # noqa: E402
 def main() -> None:
     if not pyoracle.have_reference():
         raise SystemExit("oracle/_ref is not built: run `make -C oracle` with the reference present")
+    only = set(sys.argv[1:])
     for name, build in cases.CASES.items():
+        if only and name not in only:
+            continue
         p = build()
         d = cases.to_arrays(p)
         for t in cases.THREADS:
@@ -40,6 +45,8 @@ def main() -> None:
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
         print(f"{name}: Ne={p.Ne} Nb={p.Nb} N={p.Nphoton} loss={res.loss}")
     for name, build in cases.ERROR_CASES.items():
+        if only and name not in only:
+            continue
         p = build()
         d = cases.to_arrays(p)
         try:

@@ -169,3 +169,48 @@ def test_brute_force_neighbours_small():
             cand = others + bnds
             assert len(cand) == 1
             assert hn[e, f] == cand[0]
+
+
+def test_binding_refuses_arrays_of_the_wrong_length():
+    """The library reads Ne, Nb, Nr (times the dimension) values from every
+    pointer it is given, so the binding checks each array's length first
+    and raises instead of handing over a short buffer."""
+    vmc.load_library()
+    p = cases.p3_single_cube()
+    p.mua = p.mua[:-1]
+    with pytest.raises(ValueError, match="mua"):
+        vmc.Problem(p)
+    p = cases.p3_single_cube()
+    p.BCLNormal = np.zeros((p.Nb - 1, 3))
+    with pytest.raises(ValueError, match="BCLNormal"):
+        vmc.Problem(p)
+    p = cases.p2_doc_quad()
+    p.BCType = p.BCType[:-1]
+    with pytest.raises(ValueError, match="BCType"):
+        vmc.Problem(p)
+
+
+def test_run_desc_layout_matches_the_header():
+    """The Python mirror of vmc_run_desc has the C struct's size and field
+    offsets (a C program prints them)."""
+    import tempfile
+    src = r"""
+#include "valomc.h"
+#include <stddef.h>
+#include <stdio.h>
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(vmc_run_desc), offsetof(vmc_run_desc, model),
+         offsetof(vmc_run_desc, first_photon), offsetof(vmc_run_desc, weight0),
+         offsetof(vmc_run_desc, c0), sizeof(vmc_batch_item));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        got = [int(x) for x in subprocess.check_output([exe]).split()]
+    R = vmc.RunDesc
+    want = [C.sizeof(R), R.model.offset, R.first_photon.offset, R.weight0.offset, R.c0.offset,
+            C.sizeof(vmc.BatchItem)]
+    assert got == want, (got, want)

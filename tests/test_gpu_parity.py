@@ -41,15 +41,22 @@ def rel_l2(a, b):
 
 def _run(vmc, p, **kw):
     with vmc.Problem(p) as prob:
-        return prob.run(kw.pop("nphoton", p.Nphoton), seed=kw.pop("seed", p.seed), f=p.f, phase0=p.phase0, **kw)
+        return prob.run(kw.pop("nphoton", p.Nphoton), seed=kw.pop("seed", p.seed), f=p.f, phase0=p.phase0,
+                        weight0=p.weight0, chance=p.chance, c0=p.c0, **kw)
 
 
+@pytest.mark.parametrize("tally", ["fp64", "fixed"])
 @pytest.mark.parametrize("name", sorted(cases.CASES))
-def test_fp64_matches_oracle_on_same_streams(vmc, name):
+def test_fp64_matches_oracle_on_same_streams(vmc, name, tally):
+    """FP64 transport against the Philox-mode oracle, photon for photon,
+    with FP64 tallies and with the bit-reproducible fixed-point tallies
+    (deposits rounded to 2^-32: far below the 1e-9 bound).  The roulette
+    cases run the reference's weight0/chance/c0 members at non-default
+    values."""
     p = _golden(name)
-    print(f"case {name} Ne={p.Ne} N={p.Nphoton}", flush=True)
+    print(f"case {name} Ne={p.Ne} N={p.Nphoton} tally={tally}", flush=True)
     ref = pyoracle.run(p, 4, pyoracle.RNG_PHILOX)
-    res = _run(vmc, p, precision=vmc.FP64)
+    res = _run(vmc, p, precision=vmc.FP64, tally=vmc.TALLY_FIXED64 if tally == "fixed" else vmc.TALLY_FP64)
     for k in OUT:
         a, b = getattr(res, k), getattr(ref, k)
         if np.linalg.norm(b) == 0:

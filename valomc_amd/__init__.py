@@ -37,6 +37,7 @@ MCERRORS = (
 STATUS_UNDEFINED_DIRECTION = 17
 STATUS_INVALID_ARGUMENT = 18
 STATUS_ABORTED = 103
+STATUS_TALLY_RANGE = 104
 
 
 class ProblemDesc(C.Structure):
@@ -56,9 +57,14 @@ class RunDesc(C.Structure):
         ("precision", C.c_int32), ("tally", C.c_int32), ("block_size", C.c_int32),
         ("blocks_per_cu", C.c_int32), ("replicas", C.c_int32), ("refill_min", C.c_int32),
         ("stream_tag", C.c_uint32), ("lds_slots", C.c_int32), ("direct_flush", C.c_int32),
-        ("chunk", C.c_int32), ("cache_epoch", C.c_int32), ("reserved", C.c_int32 * 1),
-        ("first_photon", C.c_uint64),
+        ("chunk", C.c_int32), ("cache_epoch", C.c_int32), ("model", C.c_int32),
+        ("first_photon", C.c_uint64), ("weight0", C.c_double), ("chance", C.c_double), ("c0", C.c_double),
     ]
+
+
+# the reference's model constants (MC3D.hpp:178-183): roulette threshold,
+# roulette survival chance, speed of light in mm/s
+WEIGHT0, CHANCE, C0 = 0.001, 0.1, 2.99792458e11
 
 
 class Output(C.Structure):
@@ -155,6 +161,17 @@ def _colmajor(a, dtype):
     return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
 
 
+def _sized(a, dtype, n: int, name: str):
+    """``a`` as a column-major buffer of exactly ``n`` values; the library
+    reads that many, so a shorter or longer array is refused here."""
+    if isinstance(a, (str, bytes)):
+        a = np.frombuffer(a.encode() if isinstance(a, str) else a, dtype=np.uint8)
+    arr = _colmajor(a, dtype)
+    if arr.size != n:
+        raise ValueError(f"{name} has {arr.size} values, expected {n}")
+    return arr
+
+
 @dataclass
 class RunResult:
     """Outputs of one solve (normalised as the reference leaves them, in the
@@ -181,8 +198,12 @@ def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float 
                   precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
                   blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
                   lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0,
-                  cache_epoch: int = 0, first_photon: int = 0) -> RunDesc:
-    """vmc_run_desc with the C ABI's field meanings (include/valomc.h)."""
+                  cache_epoch: int = 0, first_photon: int = 0, weight0: Optional[float] = None,
+                  chance: Optional[float] = None, c0: Optional[float] = None) -> RunDesc:
+    """vmc_run_desc with the C ABI's field meanings (include/valomc.h).
+    ``weight0``, ``chance`` and ``c0`` are the model constants (MC3D::weight0,
+    chance, c0); giving any of them sets ``model`` = 1 with the reference's
+    defaults for the others."""
     rd = RunDesc()
     rd.f, rd.phase0, rd.Nphoton, rd.seed = float(f), float(phase0), int(nphoton), int(seed) & (2**64 - 1)
     rd.precision, rd.tally = int(precision), int(tally)
@@ -190,6 +211,11 @@ def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float 
     rd.refill_min, rd.stream_tag, rd.lds_slots = int(refill_min), int(stream_tag), int(lds_slots)
     rd.direct_flush, rd.chunk, rd.cache_epoch = int(direct_flush), int(chunk), int(cache_epoch)
     rd.first_photon = int(first_photon)
+    if weight0 is not None or chance is not None or c0 is not None:
+        rd.model = 1
+        rd.weight0 = WEIGHT0 if weight0 is None else float(weight0)
+        rd.chance = CHANCE if chance is None else float(chance)
+        rd.c0 = C0 if c0 is None else float(c0)
     return rd
 
 
@@ -212,25 +238,29 @@ class Problem:
         """`p` has the fields of ``meshgen.Problem`` (row-per-element arrays)."""
         self.lib = load_library()
         self.dim, self.Ne, self.Nb, self.Nr = int(p.dim), int(p.Ne), int(p.Nb), int(p.Nr)
+        if self.dim not in (2, 3):
+            raise ValueError(f"dim must be 2 or 3, not {self.dim}")
+        Ne, Nb, Nr, nv = self.Ne, self.Nb, self.Nr, self.dim + 1
         keep = []
 
-        def ptr(a, dtype):
+        def ptr(a, dtype, n, name):
             if a is None:
                 return None
-            arr = _colmajor(a, dtype)
+            arr = _sized(a, dtype, n, name)
             keep.append(arr)
             return arr.ctypes.data
 
         d = ProblemDesc()
-        d.dim, d.Ne, d.Nb, d.Nr = self.dim, self.Ne, self.Nb, self.Nr
-        d.H, d.HN, d.BH, d.r = ptr(p.H, np.int64), ptr(p.HN, np.int64), ptr(p.BH, np.int64), ptr(p.r, np.float64)
-        d.mua, d.mus = ptr(p.mua, np.float64), ptr(p.mus, np.float64)
-        d.g, d.n = ptr(p.g, np.float64), ptr(p.n, np.float64)
-        d.BCType, d.BCn = ptr(p.BCType, np.uint8), ptr(p.BCn, np.float64)
-        d.BCLNormal = ptr(getattr(p, "BCLNormal", None), np.float64)
-        d.BCLightDirectionType = ptr(getattr(p, "BCLDirType", None), np.uint8)
-        d.BCIntensity = ptr(getattr(p, "BCIntensity", None), np.float64)
-        d.GaussianSigma = ptr(getattr(p, "GaussianSigma", None), np.float64)
+        d.dim, d.Ne, d.Nb, d.Nr = self.dim, Ne, Nb, Nr
+        d.H, d.HN = ptr(p.H, np.int64, Ne * nv, "H"), ptr(p.HN, np.int64, Ne * nv, "HN")
+        d.BH, d.r = ptr(p.BH, np.int64, Nb * self.dim, "BH"), ptr(p.r, np.float64, Nr * self.dim, "r")
+        d.mua, d.mus = ptr(p.mua, np.float64, Ne, "mua"), ptr(p.mus, np.float64, Ne, "mus")
+        d.g, d.n = ptr(p.g, np.float64, Ne, "g"), ptr(p.n, np.float64, Ne, "n")
+        d.BCType, d.BCn = ptr(p.BCType, np.uint8, Nb, "BCType"), ptr(p.BCn, np.float64, Nb, "BCn")
+        d.BCLNormal = ptr(getattr(p, "BCLNormal", None), np.float64, Nb * self.dim, "BCLNormal")
+        d.BCLightDirectionType = ptr(getattr(p, "BCLDirType", None), np.uint8, Nb, "BCLightDirectionType")
+        d.BCIntensity = ptr(getattr(p, "BCIntensity", None), np.float64, Nb, "BCIntensity")
+        d.GaussianSigma = ptr(getattr(p, "GaussianSigma", None), np.float64, Nb, "GaussianSigma")
         h = C.c_void_p()
         st = self.lib.vmc_problem_create(C.byref(d), C.byref(h))
         if st:
@@ -259,19 +289,21 @@ class Problem:
         """vmc_problem_create_variant: same mesh, other light sources and/or
         optical parameters (None keeps this problem's)."""
         keep = []
+        Ne, Nb = self.Ne, self.Nb
 
-        def ptr(a, dtype):
+        def ptr(a, dtype, n, name):
             if a is None:
                 return None
-            if isinstance(a, (str, bytes)):
-                a = np.frombuffer(a.encode() if isinstance(a, str) else a, dtype=np.uint8)
-            arr = _colmajor(a, dtype)
+            arr = _sized(a, dtype, n, name)
             keep.append(arr)
             return arr.ctypes.data
 
-        d = VariantDesc(ptr(mua, np.float64), ptr(mus, np.float64), ptr(g, np.float64), ptr(n, np.float64),
-                        ptr(BCType, np.uint8), ptr(BCLNormal, np.float64), ptr(BCLDirType, np.uint8),
-                        ptr(BCIntensity, np.float64), ptr(GaussianSigma, np.float64))
+        d = VariantDesc(ptr(mua, np.float64, Ne, "mua"), ptr(mus, np.float64, Ne, "mus"),
+                        ptr(g, np.float64, Ne, "g"), ptr(n, np.float64, Ne, "n"),
+                        ptr(BCType, np.uint8, Nb, "BCType"), ptr(BCLNormal, np.float64, Nb * self.dim, "BCLNormal"),
+                        ptr(BCLDirType, np.uint8, Nb, "BCLightDirectionType"),
+                        ptr(BCIntensity, np.float64, Nb, "BCIntensity"),
+                        ptr(GaussianSigma, np.float64, Nb, "GaussianSigma"))
         h = C.c_void_p()
         st = self.lib.vmc_problem_create_variant(self._h, C.byref(d), C.byref(h))
         if st:
@@ -299,7 +331,8 @@ class Problem:
     def set_medium(self, mua=None, mus=None, g=None, n=None):
         """vmc_problem_set_medium: new optical parameters per element (None
         keeps the current ones) on the resident mesh."""
-        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (mua, mus, g, n)]
+        arrs = [None if a is None else _sized(a, np.float64, self.Ne, name)
+                for a, name in ((mua, "mua"), (mus, "mus"), (g, "g"), (n, "n"))]
         st = self.lib.vmc_problem_set_medium(self._h, *[None if a is None else a.ctypes.data for a in arrs])
         if st:
             raise VmcError(st, self.lib)
@@ -308,7 +341,8 @@ class Problem:
             precision: int = FP64, tally: int = TALLY_FP64, block_size: int = 0,
             blocks_per_cu: int = 0, replicas: int = 0, refill_min: int = 0, stream_tag: int = 0,
             lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0, cache_epoch: int = 0,
-            first_photon: int = 0,
+            first_photon: int = 0, weight0: Optional[float] = None, chance: Optional[float] = None,
+            c0: Optional[float] = None,
             progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
         """vmc_problem_run, MonteCarlo() of the reference (MC3D.hpp:1641-1822):
         transport ``nphoton`` photons (stream ids first_photon ..
@@ -316,8 +350,10 @@ class Problem:
         normalised ER/EI/EBR/EBI/DEBR/DEBI in the caller's element order with
         the run's statistics.  ``precision`` FP64 or FP32 (FP32 transport,
         FP64 tallies); ``tally`` TALLY_FP64 or TALLY_FIXED64 (bit-reproducible);
-        the remaining keywords are the launch knobs of vmc_run_desc (0 =
-        default), which never change fixed-point results.  ``progress(percent)``
+        ``weight0``, ``chance``, ``c0`` are the model constants (None: the
+        reference's 0.001, 0.1, 2.99792458e11); the remaining keywords are
+        the launch knobs of vmc_run_desc (0 = default), which never change
+        fixed-point results.  ``progress(percent)``
         returning False aborts (status STATUS_ABORTED, outputs normalised by
         the photons planned, as the reference does); ``check`` raises VmcError
         on any other non-zero status."""
@@ -325,7 +361,7 @@ class Problem:
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
                            direct_flush=direct_flush, chunk=chunk, cache_epoch=cache_epoch,
-                           first_photon=first_photon)
+                           first_photon=first_photon, weight0=weight0, chance=chance, c0=c0)
         out, o = self._outputs()
         s = Stats()
         cb = None
@@ -593,8 +629,8 @@ class _EngineBase:
         if self._problem is None:
             self.Init()
         res = self._problem.run(self.Nphoton, seed=self.seed, f=self.f, phase0=self.phase0,
-                                precision=self.precision, tally=self.tally, progress=progress,
-                                check=False)
+                                precision=self.precision, tally=self.tally, weight0=self.weight0,
+                                chance=self.chance, c0=self.c0, progress=progress, check=False)
         if res.status not in (0, STATUS_ABORTED):
             raise VmcError(res.status, self._problem.lib)
         for k in ("ER", "EI", "EBR", "EBI", "DEBR", "DEBI"):

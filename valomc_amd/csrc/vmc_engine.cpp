@@ -13,6 +13,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 
 namespace vmc {
 
@@ -22,12 +23,30 @@ namespace vmc {
 // ---------------------------------------------------------------------------
 
 
-// Validates a run's precision, tally and tuning fields and resolves the
-// launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of LDS.
+int run_model(const vmc_run_desc *run, RunModel *m) {
+  *m = RunModel{};
+  if (run->model == 0) return VMC_OK;
+  if (run->model != 1) return VMC_ERR_INVALID_ARGUMENT;
+  if (!(std::isfinite(run->weight0) && run->weight0 >= 0.0) || !(run->chance > 0.0 && run->chance <= 1.0) ||
+      !(std::isfinite(run->c0) && run->c0 > 0.0))
+    return VMC_ERR_INVALID_ARGUMENT;
+  m->weight0 = run->weight0;
+  m->chance = run->chance;
+  m->c0 = run->c0;
+  return VMC_OK;
+}
+
+// Validates a run's precision, tally, model and tuning fields and resolves
+// the launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of
+// LDS.
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c) {
   (void)num_cus;
   if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
   if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
+  RunModel mdl;
+  if (run_model(run, &mdl) != VMC_OK) return VMC_ERR_INVALID_ARGUMENT;
+  c->weight0 = mdl.weight0;
+  c->chance = mdl.chance;
   c->dim = dim;
   c->f64 = run->precision == VMC_FP64;
   c->fixed = run->tally == VMC_TALLY_FIXED64;
@@ -161,8 +180,8 @@ WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<I
   // a batch's items follow one another through the launch, so the cache
   // slots are handed back every 1024 passes for the items walked next
   P.cache_epoch = c.epoch > 0 ? c.epoch : (c.epoch == 0 && w.batch && c.slots > 0 ? 1024 : 0);
-  P.weight0 = 0.001;   // MC3D.hpp:182
-  P.chance = 0.1;      // MC3D.hpp:183
+  P.weight0 = c.weight0;   // MC3D::weight0, chance (MC3D.hpp:160, defaults :182-183)
+  P.chance = c.chance;
   P.weight0_f = (float)P.weight0;
   P.chance_f = (float)P.chance;
   P.fix_scale = 4294967296.0;
@@ -175,13 +194,23 @@ void enqueue_walk(Arena &a, const LaunchCfg &c, const WalkLaunch &w) {
   hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, w.batch, w.grid, c.block, c.lds, a.stream, w.P), "walk");
 }
 
+void check_dispenser(unsigned long long dispensed, const WalkLaunch &w) {
+  // every wave leaves the walk only after drawing an id at or past vtotal,
+  // so a dispenser below it means photons were left unwalked
+  if (w.grid > 0 && dispensed < w.P.vtotal)
+    throw HipFail{hipErrorLaunchFailure, "walk ended before its photon dispenser reached the launch's photon count"};
+}
+
 float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out) {
   const WalkLaunch w = prepare_walk(a, num_cus, c, items);
   if (w.grid == 0) return 0.f;
   hip_check(hipEventRecord(a.ev0, a.stream), "event");
   enqueue_walk(a, c, w);
   hip_check(hipEventRecord(a.ev1, a.stream), "event");
-  hip_check(hipEventSynchronize(a.ev1), "sync");
+  unsigned long long dispensed = 0;
+  hip_check(hipMemcpyAsync(&dispensed, a.counters.p, 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipStreamSynchronize(a.stream), "sync");
+  check_dispenser(dispensed, w);
   float ms = 0.f;
   hip_check(hipEventElapsedTime(&ms, a.ev0, a.ev1), "event");
   if (grid_out) *grid_out = w.grid;
@@ -196,7 +225,7 @@ void arena_reduce(Arena &a, const LaunchCfg &c) {
 }
 
 void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                    double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss) {
   const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
   raw_e.resize(ew);
   raw_b.resize(bw);
@@ -204,14 +233,14 @@ void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, st
   hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, ew * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, bw * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(led, a.ledger.p, 5 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
-  hip_check(hipMemcpyAsync(cnt, a.counters.p, 4 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(cnt, a.counters.p, kCounters * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   if (a.nitems)
     hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, a.nitems * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipStreamSynchronize(a.stream), "sync");
 }
 
 void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss) {
+                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss) {
   arena_reduce(a, c);
   arena_copy_out(a, c, raw_e, raw_b, led, cnt, loss);
 }
@@ -234,7 +263,9 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
   it.tag = run->stream_tag;
   double phase0 = run->phase0;
   if (phase0 < 0) phase0 += 2 * M_PI * ceil(-phase0 / (2 * M_PI));   // MC3D.cpp:124-126
-  it.kfac = 2.0 * M_PI * run->f / kC0;
+  RunModel mdl;
+  (void)run_model(run, &mdl);   // validated by resolve_cfg
+  it.kfac = 2.0 * M_PI * run->f / mdl.c0;
   it.cph0 = cos(phase0);
   it.sph0 = sin(phase0);
   return it;
@@ -243,10 +274,12 @@ ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint6
 // Splits a problem's raw element words (device order, from slot tbase) and
 // boundary words (from slot bbase) and writes normalised outputs
 // (MC3D.hpp:1793-1818).
-void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
-                    vmc_output *out) {
+void finish_outputs(const vmc_problem &m, const vmc_run_desc &run, int64_t N, int W, const double *raw_e,
+                    const double *raw_b, vmc_output *out) {
   if (!out || (!out->ER && !out->EI && !out->EBR && !out->EBI && !out->DEBR && !out->DEBI)) return;
-  normalise_into(m, 2.0 * M_PI * f, N, W, raw_e, raw_b, out);
+  RunModel mdl;
+  (void)run_model(&run, &mdl);
+  normalise_into(m, 2.0 * M_PI * run.f, mdl.c0, N, W, raw_e, raw_b, out);
 }
 
 int64_t sector_round(int64_t n) { return (n + kSectorElems - 1) / kSectorElems * kSectorElems; }
@@ -315,38 +348,55 @@ struct BatchGroup {
 };
 
 // Checks a batch (problems on one device, one precision and tally mode),
-// splits it into groups in launch order with the settings of items[0].run,
-// and uploads the problems' element records; a status.
+// splits it into groups in launch order -- one per (dimension, CW/FD,
+// roulette constants) -- with the settings of items[0].run, and uploads the
+// problems' element records; a status.
 int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &groups) {
   if (!items || n <= 0 || !items[0].prob) return VMC_ERR_INVALID_ARGUMENT;
   const vmc_problem *m0 = items[0].prob;
+  std::vector<RunModel> model((size_t)n);
   for (int32_t i = 0; i < n; ++i) {
     const vmc_batch_item &b = items[i];
     if (!b.prob || b.run.Nphoton < 0 || b.prob->device != m0->device) return VMC_ERR_INVALID_ARGUMENT;
     if (b.run.precision != items[0].run.precision || b.run.tally != items[0].run.tally)
       return VMC_ERR_INVALID_ARGUMENT;
+    if (b.run.tally == VMC_TALLY_FIXED64 && b.run.Nphoton >= kFixedMaxPhotons) return VMC_ERR_INVALID_ARGUMENT;
+    if (run_model(&b.run, &model[i]) != VMC_OK) return VMC_ERR_INVALID_ARGUMENT;
   }
   try {
     hip_check(hipSetDevice(m0->device), "hipSetDevice");
     for (int dim = 2; dim <= 3; ++dim) {
       for (int fdi = 0; fdi < 2; ++fdi) {
-        BatchGroup g;
-        for (int32_t i = 0; i < n; ++i)
-          if (items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1))) g.idx.push_back(i);
-        if (g.idx.empty()) continue;
-        const int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &g.c);
-        if (st != VMC_OK) return st;
-        for (int32_t i : g.idx) {
-          vmc_problem &m = *items[i].prob;
-          upload_params(m, g.c.f64);
-          g.tb.push_back(g.ne);
-          g.bb.push_back(g.nb);
-          g.ne += sector_round(m.Ne);
-          g.nb += m.Nb;
+        auto in_class = [&](int32_t i) {
+          return items[i].prob->dim == dim && ((2.0 * M_PI * items[i].run.f > 0.0) == (fdi == 1));
+        };
+        // the roulette constants are launch-wide: one group per pair, in
+        // order of first appearance
+        std::vector<std::pair<double, double>> pairs;
+        for (int32_t i = 0; i < n; ++i) {
+          const std::pair<double, double> pr(model[i].weight0, model[i].chance);
+          if (in_class(i) && std::find(pairs.begin(), pairs.end(), pr) == pairs.end()) pairs.push_back(pr);
         }
-        if (g.ne > ((int64_t)1 << 30) / g.c.W || g.nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
-        choose_replicas(g.c, &items[0].run, g.ne, g.nb);
-        groups.push_back(std::move(g));
+        for (const auto &pr : pairs) {
+          BatchGroup g;
+          for (int32_t i = 0; i < n; ++i)
+            if (in_class(i) && model[i].weight0 == pr.first && model[i].chance == pr.second) g.idx.push_back(i);
+          const int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &g.c);
+          if (st != VMC_OK) return st;
+          g.c.weight0 = pr.first;
+          g.c.chance = pr.second;
+          for (int32_t i : g.idx) {
+            vmc_problem &m = *items[i].prob;
+            upload_params(m, g.c.f64);
+            g.tb.push_back(g.ne);
+            g.bb.push_back(g.nb);
+            g.ne += sector_round(m.Ne);
+            g.nb += m.Nb;
+          }
+          if (g.ne > ((int64_t)1 << 30) / g.c.W || g.nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
+          choose_replicas(g.c, &items[0].run, g.ne, g.nb);
+          groups.push_back(std::move(g));
+        }
       }
     }
   } catch (HipFail f) {
@@ -358,11 +408,11 @@ int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &
 // Normalised outputs and loss counts of a group's items from its raw words;
 // adds the group's counters and ledger to *tot.
 void finish_group(vmc_batch_item *items, const BatchGroup &g, const std::vector<double> &raw_e,
-                  const std::vector<double> &raw_b, const double led[5], const unsigned long long cnt[4],
+                  const std::vector<double> &raw_b, const double led[5], const unsigned long long cnt[kCounters],
                   const std::vector<unsigned long long> &loss, vmc_stats *tot) {
   for (size_t k = 0; k < g.idx.size(); ++k) {
     vmc_batch_item &b = items[g.idx[k]];
-    finish_outputs(*b.prob, b.run.f, b.run.Nphoton, g.c.W, raw_e.data() + g.tb[k] * g.c.W,
+    finish_outputs(*b.prob, b.run, b.run.Nphoton, g.c.W, raw_e.data() + g.tb[k] * g.c.W,
                    raw_b.data() + g.bb[k] * 4, &b.out);
     b.loss = (int64_t)loss[k];
     tot->photons += b.run.Nphoton;
@@ -374,6 +424,71 @@ void finish_group(vmc_batch_item *items, const BatchGroup &g, const std::vector<
   tot->block_size = g.c.block;
 }
 
+
+// ---------------------------------------------------------------------------
+// One problem's MonteCarlo() (vmc_problem_run)
+// ---------------------------------------------------------------------------
+
+int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
+                vmc_progress_fn progress, void *user) {
+  if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
+  const auto t_start = std::chrono::steady_clock::now();
+  const bool fd = 2.0 * M_PI * run->f > 0.0;
+  LaunchCfg c;
+  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c);
+  if (st != VMC_OK) return st;
+  const int64_t N = run->Nphoton;
+  if (c.fixed && N >= kFixedMaxPhotons) return VMC_ERR_INVALID_ARGUMENT;
+  int aborted = 0, grid = 0;
+  float kernel_ms = 0.f;
+  double led[5] = {0, 0, 0, 0, 0};
+  unsigned long long cnt[kCounters] = {};
+  std::vector<double> raw_e, raw_b;
+  std::vector<unsigned long long> loss;
+  const int64_t ne = sector_round(m->Ne), nb = m->Nb;
+  try {
+    hip_check(hipSetDevice(m->device), "hipSetDevice");
+    upload_params(*m, c.f64);
+    choose_replicas(c, run, ne, nb);
+    Arena &a = m->arena;
+    arena_reset(a, c, ne, nb, 1);
+    // photon ranges: one launch, or ~20 when a progress callback wants ticks
+    const int nseg = progress ? 20 : 1;
+    int64_t done = 0;
+    for (int sgm = 0; sgm < nseg && done < N; ++sgm) {
+      const int64_t end = (sgm == nseg - 1) ? N : (N * (sgm + 1)) / nseg;
+      if (end <= done) continue;
+      std::vector<ItemRec> items{make_item(*m, run, c.f64, (uint64_t)(end - done), (uint64_t)done, 0, 0)};
+      kernel_ms += launch_items(a, m->num_cus, c, items, &grid);
+      done = end;
+      if (progress && !progress(100.0 * (double)done / (double)(N > 0 ? N : 1), user)) {
+        aborted = 1;
+        break;
+      }
+    }
+    arena_collect(a, c, raw_e, raw_b, led, cnt, loss);
+  } catch (HipFail f) {
+    return fail_hip(f);
+  }
+  // like the reference, an aborted run still divides by the planned N
+  finish_outputs(*m, *run, N, c.W, raw_e.data(), raw_b.data(), out);
+  const bool out_of_range = c.fixed && cnt[4] != 0;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->photons = N;
+    stats->steps = (int64_t)cnt[1];
+    stats->loss = (int64_t)cnt[2];
+    stats->rechecks = (int64_t)cnt[3];
+    stats->kernel_ms = kernel_ms;
+    for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
+    stats->grid_blocks = grid;
+    stats->block_size = c.block;
+    stats->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  }
+  if (out_of_range) return range_error();
+  return aborted ? VMC_ERR_ABORTED : VMC_OK;
+}
 
 } // namespace vmc
 
@@ -391,6 +506,7 @@ struct vmc_batch_plan {
   std::vector<vmc_problem *> probs;
   std::vector<int64_t> counts;
   std::vector<bool> fd;
+  std::vector<vmc::RunModel> model;
   std::vector<std::unique_ptr<Group>> groups;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -565,58 +681,7 @@ int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) try {
 
 int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
                     vmc_progress_fn progress, void *user) try {
-  if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
-  const auto t_start = std::chrono::steady_clock::now();
-  const bool fd = 2.0 * M_PI * run->f > 0.0;
-  LaunchCfg c;
-  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c);
-  if (st != VMC_OK) return st;
-  const int64_t N = run->Nphoton;
-  int aborted = 0, grid = 0;
-  float kernel_ms = 0.f;
-  double led[5] = {0, 0, 0, 0, 0};
-  unsigned long long cnt[4] = {0, 0, 0, 0};
-  std::vector<double> raw_e, raw_b;
-  std::vector<unsigned long long> loss;
-  try {
-    hip_check(hipSetDevice(m->device), "hipSetDevice");
-    upload_params(*m, c.f64);
-    choose_replicas(c, run, sector_round(m->Ne), m->Nb);
-    arena_reset(m->arena, c, sector_round(m->Ne), m->Nb, 1);
-    // photon ranges: one launch, or ~20 when a progress callback wants ticks
-    const int nseg = progress ? 20 : 1;
-    int64_t done = 0;
-    for (int sgm = 0; sgm < nseg && done < N; ++sgm) {
-      const int64_t end = (sgm == nseg - 1) ? N : (N * (sgm + 1)) / nseg;
-      if (end <= done) continue;
-      std::vector<ItemRec> items{make_item(*m, run, c.f64, (uint64_t)(end - done), (uint64_t)done, 0, 0)};
-      kernel_ms += launch_items(m->arena, m->num_cus, c, items, &grid);
-      done = end;
-      if (progress && !progress(100.0 * (double)done / (double)(N > 0 ? N : 1), user)) {
-        aborted = 1;
-        break;
-      }
-    }
-    arena_collect(m->arena, c, raw_e, raw_b, led, cnt, loss);
-  } catch (HipFail f) {
-    return fail_hip(f);
-  }
-  // like the reference, an aborted run still divides by the planned N
-  finish_outputs(*m, run->f, N, c.W, raw_e.data(), raw_b.data(), out);
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
-    stats->photons = N;
-    stats->steps = (int64_t)cnt[1];
-    stats->loss = (int64_t)cnt[2];
-    stats->rechecks = (int64_t)cnt[3];
-    stats->kernel_ms = kernel_ms;
-    for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
-    stats->grid_blocks = grid;
-    stats->block_size = c.block;
-    stats->total_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-  }
-  return aborted ? VMC_ERR_ABORTED : VMC_OK;
+  return problem_run(m, run, out, stats, progress, user);
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
 }
@@ -657,17 +722,17 @@ int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_frac
     b.out = vmc_output{};
   }
   return autotune_knobs(items[0].run, tuned, trials, [&](vmc_run_desc cand) {
+    // the candidate's launch knobs on every item; each item keeps its own
+    // photons, stream, phase and model constants
     for (vmc_batch_item &b : trial) {
-      const int64_t keep = b.run.Nphoton;
-      const uint64_t seed = b.run.seed;
-      const uint32_t tag = b.run.stream_tag;
-      const double f = b.run.f, ph = b.run.phase0;
-      b.run = cand;
-      b.run.Nphoton = keep;
-      b.run.seed = seed;
-      b.run.stream_tag = tag;
-      b.run.f = f;
-      b.run.phase0 = ph;
+      b.run.block_size = cand.block_size;
+      b.run.blocks_per_cu = cand.blocks_per_cu;
+      b.run.replicas = cand.replicas;
+      b.run.refill_min = cand.refill_min;
+      b.run.lds_slots = cand.lds_slots;
+      b.run.direct_flush = cand.direct_flush;
+      b.run.chunk = cand.chunk;
+      b.run.cache_epoch = cand.cache_epoch;
     }
     vmc_stats s{};
     return vmc_batch_run(trial.data(), n, &s) == VMC_OK ? s.kernel_ms : std::numeric_limits<double>::infinity();
@@ -690,6 +755,7 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
   vmc_problem *m0 = items[0].prob;
   vmc_stats tot{};
   float kernel_ms = 0.f;
+  bool range_bad = false;
   try {
     std::lock_guard<std::mutex> lock(g_batch_mutex);
     if (!g_batch_arena.stream) g_batch_arena.open();
@@ -705,10 +771,11 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
       std::vector<double> raw_e, raw_b;
       std::vector<unsigned long long> loss;
       double led[5];
-      unsigned long long cnt[4];
+      unsigned long long cnt[kCounters];
       arena_collect(g_batch_arena, g.c, raw_e, raw_b, led, cnt, loss);
       finish_group(items, g, raw_e, raw_b, led, cnt, loss, &tot);
       tot.grid_blocks = grid;
+      if (g.c.fixed && cnt[4]) range_bad = true;
     }
   } catch (HipFail f) {
     return fail_hip(f);
@@ -719,7 +786,7 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
-  return VMC_OK;
+  return range_bad ? range_error() : VMC_OK;
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
 }
@@ -741,6 +808,9 @@ int vmc_batch_plan_create(const vmc_batch_item *items, int32_t n, vmc_batch_plan
     pl->probs.push_back(items[i].prob);
     pl->counts.push_back(items[i].run.Nphoton);
     pl->fd.push_back(2.0 * M_PI * items[i].run.f > 0.0);
+    RunModel m;
+    (void)run_model(&items[i].run, &m);   // validated by plan_batch
+    pl->model.push_back(m);
   }
   try {
     hip_check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -789,8 +859,10 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
   if (!pl || !items || n != (int32_t)pl->probs.size()) return VMC_ERR_INVALID_ARGUMENT;
   for (int32_t i = 0; i < n; ++i) {
     const vmc_run_desc &r = items[i].run;
+    RunModel m;
     if (items[i].prob != pl->probs[i] || r.Nphoton != pl->counts[i] || r.precision != pl->precision ||
-        r.tally != pl->tally || (2.0 * M_PI * r.f > 0.0) != pl->fd[i])
+        r.tally != pl->tally || (2.0 * M_PI * r.f > 0.0) != pl->fd[i] || run_model(&r, &m) != VMC_OK ||
+        m.weight0 != pl->model[i].weight0 || m.chance != pl->model[i].chance)
       return VMC_ERR_INVALID_ARGUMENT;
     items[i].loss = 0;
     items[i].status = VMC_OK;
@@ -798,6 +870,7 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
   const auto t_start = std::chrono::steady_clock::now();
   vmc_stats tot{};
   float kernel_ms = 0.f;
+  bool range_bad = false;
   try {
     hip_check(hipSetDevice(pl->device), "hipSetDevice");
     // current records: seeds, tags, phases and the problems' current media
@@ -823,10 +896,12 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
       std::vector<double> raw_e, raw_b;
       std::vector<unsigned long long> loss;
       double led[5];
-      unsigned long long cnt[4];
+      unsigned long long cnt[kCounters];
       arena_copy_out(pg->arena, pg->g.c, raw_e, raw_b, led, cnt, loss);
+      check_dispenser(cnt[0], pg->w);
       finish_group(items, pg->g, raw_e, raw_b, led, cnt, loss, &tot);
       tot.grid_blocks = pg->w.grid;
+      if (pg->g.c.fixed && cnt[4]) range_bad = true;
     }
   } catch (HipFail f) {
     return fail_hip(f);
@@ -837,7 +912,7 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }
-  return VMC_OK;
+  return range_bad ? range_error() : VMC_OK;
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
 }
@@ -920,6 +995,7 @@ const char *vmc_status_string(int st) {
   case VMC_ERR_OUT_OF_MEMORY: return "Out of memory (device, or host: see vmc_last_error)";
   case VMC_ERR_NO_DEVICE: return "No GPU device";
   case VMC_ERR_ABORTED: return "Computation aborted by the progress callback";
+  case VMC_ERR_TALLY_RANGE: return "A fixed-point tally word left its exact range";
   default: return "Undefined error";
   }
 }

@@ -48,6 +48,15 @@ constexpr int kFace3[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}}; // MC
 constexpr int kEdge2[3][2] = {{0, 1}, {1, 2}, {2, 0}};                     // MC2D.hpp:921
 constexpr int64_t kUnset = INT64_MAX;                                      // MC3D.hpp:29-31
 constexpr double kC0 = 2.99792458e11;                                      // MC3D.hpp:178
+constexpr double kWeight0 = 0.001, kChance = 0.1;                          // MC3D.hpp:182-183
+
+// Model constants of a run (vmc_run_desc.model; the reference's members
+// weight0, chance and c0)
+struct RunModel {
+  double weight0 = kWeight0, chance = kChance, c0 = kC0;
+};
+// the run's constants, validated; a status
+int run_model(const vmc_run_desc *run, RunModel *m);
 
 struct HipFail {
   hipError_t e;
@@ -193,7 +202,7 @@ void build_boundary(vmc_problem &m);
 void upload_params(vmc_problem &m, bool f64);
 // normalisation of MonteCarlo (MC3D.hpp:1793-1818, MC2D.hpp:1673-1703),
 // from raw device-order words straight into the caller's outputs
-void normalise_into(const vmc_problem &m, double omega, int64_t N, int W, const double *raw_e,
+void normalise_into(const vmc_problem &m, double omega, double c0, int64_t N, int W, const double *raw_e,
                     const double *raw_b, vmc_output *out);
 // a HIP failure as a status, with its text in g_last_error
 int fail_hip(const HipFail &f);
@@ -214,6 +223,22 @@ template <class T> void copy_in(std::vector<T> &dst, const T *src, int64_t n) {
 // ---- launch engine (vmc_engine.cpp) ----
 
 constexpr uint64_t kChunk = 128;   // default photon ids a wave takes from the dispenser at once
+// device counters of a launch: [0] photon dispenser, [1] element steps,
+// [2] lost photons, [3] FP32 exit faces re-decided in FP64, [4] fixed-point
+// range flag
+constexpr int kCounters = 8;
+// fixed-point tallies: runs are limited to fewer photons than this, so the
+// int64 sums of exitance and absorbed weight (at most about one unit per
+// photon and word) stay below 2^31 * 2^32
+constexpr int64_t kFixedMaxPhotons = (int64_t)1 << 31;
+// VMC_ERR_TALLY_RANGE with its text
+inline int range_error() {
+  g_last_error = "a fixed-point tally deposit reached 2^30 (or was not finite): the int64 sums would not be exact";
+  return VMC_ERR_TALLY_RANGE;
+}
+// throws when a launch's dispenser ended below its photon count
+struct WalkLaunch;
+void check_dispenser(unsigned long long dispensed, const WalkLaunch &w);
 
 struct LaunchCfg {
   int dim = 3;
@@ -224,6 +249,7 @@ struct LaunchCfg {
   int epoch = 0;   // -1: never; 0: by launch (batches 1024); > 0: passes
   int bpc = 0;     // workgroups per CU: 0 = as many as fit, else at most this many
   size_t lds = 0;
+  double weight0 = kWeight0, chance = kChance;   // roulette constants of the launch
 };
 
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c);
@@ -255,13 +281,13 @@ float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRe
 // per-item loss counts to the host and waits.
 void arena_reduce(Arena &a, const LaunchCfg &c);
 void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                    double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss);
+                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss);
 void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[4], std::vector<unsigned long long> &loss);   // reduce + copy out
+                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss);   // reduce + copy out
 ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint64_t count, uint64_t id0,
                   int64_t tbase, int64_t bbase);
-void finish_outputs(const vmc_problem &m, double f, int64_t N, int W, const double *raw_e, const double *raw_b,
-                    vmc_output *out);
+void finish_outputs(const vmc_problem &m, const vmc_run_desc &run, int64_t N, int W, const double *raw_e,
+                    const double *raw_b, vmc_output *out);
 int64_t sector_round(int64_t n);
 
 } // namespace vmc

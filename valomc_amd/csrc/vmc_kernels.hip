@@ -202,8 +202,9 @@ template <bool FIXED, int WS> struct SectorCache {
   }
   // moves the lane buffer words buf[j * stride] of `sect` into the sector's
   // slot (clearing them); false, leaving them, when the sector has no slot
+  // (fixed point: `range_bad` is raised for a word at or beyond 2^62)
   template <class R>
-  __device__ __forceinline__ bool add_from(int sect, R *buf, int stride, double scale) {
+  __device__ __forceinline__ bool add_from(int sect, R *buf, int stride, double scale, bool &range_bad) {
     const int sl = claim(sect);
     if (sl < 0) return false;
 #pragma unroll 4
@@ -211,8 +212,13 @@ template <bool FIXED, int WS> struct SectorCache {
       const R v = buf[j * stride];
       buf[j * stride] = R(0);
       if (v == R(0)) continue;
-      if constexpr (FIXED) atomicAdd(val + (size_t)sl * WS + j, (unsigned long long)__double2ll_rn((double)v * scale));
-      else atomicAdd(reinterpret_cast<double *>(val) + (size_t)sl * WS + j, (double)v);
+      if constexpr (FIXED) {
+        const double s = (double)v * scale;
+        range_bad = range_bad || !(fabs(s) < 0x1p62);
+        atomicAdd(val + (size_t)sl * WS + j, (unsigned long long)__double2ll_rn(s));
+      } else {
+        atomicAdd(reinterpret_cast<double *>(val) + (size_t)sl * WS + j, (double)v);
+      }
     }
     return true;
   }
@@ -585,9 +591,18 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       else __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   };
+  // Fixed point: a word is v * 2^32 rounded to int64; a value at or beyond
+  // 2^30 (or not finite) would leave the range in which the int64 sums stay
+  // exact, so it raises the lane's range flag, reported after the launch.
+  bool range_bad = false;
   auto to_bits = [&](double v) -> unsigned long long {
-    if constexpr (FIXED) return (unsigned long long)__double2ll_rn(v * P.fix_scale);
-    else return (unsigned long long)__double_as_longlong(v);
+    if constexpr (FIXED) {
+      const double s = v * P.fix_scale;
+      range_bad = range_bad || !(fabs(s) < 0x1p62);
+      return (unsigned long long)__double2ll_rn(s);
+    } else {
+      return (unsigned long long)__double_as_longlong(v);
+    }
   };
   // Hands the buffers of the lanes with `go` (sector `sect`) to the cache
   // (fixed point) or to the replica, and clears them.  Converged: every lane
@@ -596,7 +611,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   // lane, so a run costs one coalesced atomic request per 64-byte line and
   // the words go from the lane buffers to the replica without a copy.
   auto flush_sector = [&](bool go) {
-    const bool need = go && (!FIXED ? true : !cache.add_from(sect, sbuf, bstride, P.fix_scale));
+    const bool need = go && (!FIXED ? true : !cache.add_from(sect, sbuf, bstride, P.fix_scale, range_bad));
     uint64_t m = __ballot(need);
     if (m == 0) return;
     if (P.direct_flush) {
@@ -720,11 +735,15 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     }
     const uint64_t live = __ballot(alive);
     if (live == 0) {
-      // with cache epochs a finished wave keeps passing through the loop
-      // (doing nothing) until the whole workgroup leaves it together at an
-      // epoch; a wave whose refill drew only the padding of an item's last
-      // chunk is not finished while the dispenser still has ids
-      if (epoch == 0) break;
+      // a wave whose refill drew only the padding of an item's last chunk
+      // is not finished while the dispenser still has ids: it refills again
+      // at the next pass.  With cache epochs a finished wave keeps passing
+      // through the loop (doing nothing) until the whole workgroup leaves it
+      // together at an epoch.
+      if (epoch == 0) {
+        if (exhausted) break;
+        continue;
+      }
       if (exhausted && !wave_done) {
         wave_done = true;
         if (lane == 0) atomicAdd(&s_done, 1);
@@ -1101,6 +1120,9 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     atomicAdd(&P.counters[1], (unsigned long long)steps);
     atomicAdd(&P.counters[2], (unsigned long long)s_led[wid][4]);
     if (nrecheck) atomicAdd(&P.counters[3], (unsigned long long)nrecheck);
+  }
+  if constexpr (FIXED) {
+    if (__ballot(range_bad) != 0 && lane == 0) atomicOr(&P.counters[4], 1ull);
   }
 }
 

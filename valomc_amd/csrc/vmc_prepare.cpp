@@ -632,7 +632,7 @@ void upload_params(vmc_problem &m, bool f64) {
 // per face) become the caller's outputs in one pass, with the reference's
 // expressions.  Words the reference leaves unnormalised (EBI in CW, DEBR
 // and DEBI in 3D) are copied as they are; NULL outputs are skipped.
-void normalise_into(const vmc_problem &m, double omega, int64_t N, int W, const double *raw_e,
+void normalise_into(const vmc_problem &m, double omega, double c0, int64_t N, int W, const double *raw_e,
                     const double *raw_b, vmc_output *out) {
   const bool d3 = m.dim == 3, fd = omega > 0.0;
   if (out->ER || out->EI)
@@ -646,7 +646,7 @@ void normalise_into(const vmc_problem &m, double omega, int64_t N, int W, const 
           else er = a / (m.vol[i] * (double)N);
         } else {
           const double b = raw_e[W * dv + 1];
-          const double k = omega / kC0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
+          const double k = omega / c0 * m.med->n[i];   // k = omega / c0 * n (MC3D.hpp:660-662)
           er = (b * k + a * mua) / (pow(k, 2) + pow(mua, 2)) / (double)N / m.vol[i];
           ei = -(a * k - b * mua) / (pow(k, 2) + pow(mua, 2)) / (double)N / m.vol[i];
         }

@@ -17,8 +17,8 @@ using namespace vmc;
 //
 // A worker thread owns one stream and a persistent tally arena.  Requests
 // wait in FIFO order until their tallies fit in the arena (admission by
-// device-memory footprint); each launch then takes one (dimension, CW/FD)
-// group of the admitted requests and advances them by parts of the
+// device-memory footprint); each launch then takes one group (dimension,
+// CW/FD, roulette constants) of the admitted requests and advances them by parts of the
 // launch's photon budget, so a request that arrives while others run joins
 // at the next launch and launch times stay bounded.  Fair policy: groups in
 // turn, equal shares.  Shortest policy: the group of the request with the
@@ -38,6 +38,7 @@ struct Request {
   vmc_run_desc run{};
   vmc_output out{};
   bool fd = false;
+  RunModel model;                // roulette constants and c0 of the run
   int64_t launched = 0;          // photons handed to launches so far
   int64_t eoff = -1, ewords = 0; // element words in each arena replica
   int64_t boff = -1;             // first boundary slot
@@ -135,7 +136,7 @@ void server_finish(vmc_server *s, Request *r) {
   hip_check(hipMemcpyAsync(raw_e.data(), s->fraw_e.p, raw_e.size() * 8, hipMemcpyDeviceToHost, s->fstream), "d2h");
   hip_check(hipMemcpyAsync(raw_b.data(), s->fraw_b.p, raw_b.size() * 8, hipMemcpyDeviceToHost, s->fstream), "d2h");
   hip_check(hipStreamSynchronize(s->fstream), "sync");
-  finish_outputs(*r->prob, r->run.f, r->run.Nphoton, r->fd ? 2 : 1, raw_e.data(), raw_b.data(), &r->out);
+  finish_outputs(*r->prob, r->run, r->run.Nphoton, r->fd ? 2 : 1, raw_e.data(), raw_b.data(), &r->out);
 }
 
 void finisher_loop(vmc_server *s) {
@@ -170,12 +171,19 @@ void finisher_loop(vmc_server *s) {
   }
 }
 
-// Shortest policy: the active requests with photons left, of the group
-// (dimension, CW/FD) of the one with the least estimated remaining device
-// time, in order of that estimate (ties: earlier ticket).  A problem never
-// launched yet is estimated at its group's least known cost per photon, or
-// 0, so a new request is measured in its first launch.  Called under the
-// server's lock.
+// Requests that can share a launch: same dimension, CW/FD mode and
+// roulette constants (launch-wide kernel parameters).
+bool same_group(const Request *a, const Request *b) {
+  return a->prob->dim == b->prob->dim && a->fd == b->fd && a->model.weight0 == b->model.weight0 &&
+         a->model.chance == b->model.chance;
+}
+
+// Shortest policy: the active requests with photons left, of the group of
+// the one with the least estimated remaining device time, in order of that
+// estimate (ties: earlier ticket).  A problem never launched yet is
+// estimated at the least known cost per photon of its (dimension, CW/FD)
+// class, or 0, so a new request is measured in its first launch.  Called
+// under the server's lock.
 void pick_shortest(vmc_server *s, std::vector<Request *> &group) {
   struct Est {
     double t;
@@ -201,9 +209,26 @@ void pick_shortest(vmc_server *s, std::vector<Request *> &group) {
   std::sort(all.begin(), all.end(), [](const Est &a, const Est &b) {
     return a.t != b.t ? a.t < b.t : a.r->ticket < b.r->ticket;
   });
-  const int g0 = gid(all[0].r);
   for (const Est &e : all)
-    if (gid(e.r) == g0) group.push_back(e.r);
+    if (same_group(e.r, all[0].r)) group.push_back(e.r);
+}
+
+// Fair policy: the groups with photons left take turns, in order of their
+// oldest request; all of the chosen group's requests share the launch.
+// Called under the server's lock.
+void pick_fair(vmc_server *s, std::vector<Request *> &group) {
+  std::vector<Request *> heads;   // one request per group, oldest first
+  for (Request *r : s->active) {
+    if (r->launched >= r->run.Nphoton) continue;
+    bool seen = false;
+    for (Request *h : heads) seen = seen || same_group(h, r);
+    if (!seen) heads.push_back(r);
+  }
+  if (heads.empty()) return;
+  Request *h = heads[(size_t)s->turn % heads.size()];
+  s->turn = (int)(((size_t)s->turn + 1) % heads.size());
+  for (Request *r : s->active)
+    if (same_group(r, h) && r->launched < r->run.Nphoton) group.push_back(r);
 }
 
 void server_loop(vmc_server *s) {
@@ -240,27 +265,11 @@ void server_loop(vmc_server *s) {
         seen = s->epoch;   // nothing fits until the finisher returns ranges
         continue;
       }
-      if (s->desc.policy == VMC_SERVE_SHORTEST) {
-        pick_shortest(s, group);
-        if (!group.empty()) {
-          gdim = group[0]->prob->dim;
-          gfd = group[0]->fd;
-        }
-      } else {
-        // the next group in turn among the (dimension, CW/FD) groups with work
-        for (int k = 0; k < 4; ++k) {
-          const int g = (s->turn + k) % 4;
-          const int dim = g < 2 ? 3 : 2;
-          const bool fd = (g & 1) != 0;
-          for (Request *r : s->active)
-            if (r->prob->dim == dim && r->fd == fd && r->launched < r->run.Nphoton) group.push_back(r);
-          if (!group.empty()) {
-            gdim = dim;
-            gfd = fd;
-            s->turn = (g + 1) % 4;
-            break;
-          }
-        }
+      if (s->desc.policy == VMC_SERVE_SHORTEST) pick_shortest(s, group);
+      else pick_fair(s, group);
+      if (!group.empty()) {
+        gdim = group[0]->prob->dim;
+        gfd = group[0]->fd;
       }
     }
     LaunchCfg c;
@@ -287,6 +296,8 @@ void server_loop(vmc_server *s) {
       if (!group.empty()) {
         const int st = resolve_cfg(gdim, s->num_cus, s->lds_per_cu, gfd, &tmpl, &c);
         if (st != VMC_OK) throw HipFail{hipErrorInvalidValue, "server launch configuration"};
+        c.weight0 = group[0]->model.weight0;
+        c.chance = group[0]->model.chance;
         // fair: equal shares of the launch budget; shortest: the budget in
         // the group's order (least remaining time first), at least
         // min_slice for the first
@@ -317,8 +328,18 @@ void server_loop(vmc_server *s) {
         int grid = 0;
         const float ms = launch_items(a, s->num_cus, c, items, &grid);
         std::vector<unsigned long long> loss(items.size());
+        unsigned long long range_flag = 0;
         hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, loss.size() * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+        hip_check(hipMemcpyAsync(&range_flag, a.counters.p + 4, 8, hipMemcpyDeviceToHost, a.stream), "d2h");
         hip_check(hipStreamSynchronize(a.stream), "sync");
+        if (c.fixed && range_flag) {
+          // this launch's requests are out of the fixed-point range; the
+          // flag is cleared for the launches that follow
+          hip_check(launch_fill_u64(a.counters.p + 4, 1, 0ull, a.stream), "fill");
+          (void)range_error();
+          std::lock_guard<std::mutex> lk(s->mu);
+          for (Request *r : group) r->status = VMC_ERR_TALLY_RANGE;
+        }
         std::lock_guard<std::mutex> lk(s->mu);
         s->launches++;
         s->busy_ms += ms;
@@ -428,7 +449,11 @@ int vmc_server_create(const vmc_server_desc *d, vmc_server **out) try {
 int vmc_server_submit(vmc_server *s, vmc_problem *prob, const vmc_run_desc *run, const vmc_output *out,
                       uint64_t *ticket) try {
   if (!s || !prob || !run || !ticket || run->Nphoton < 0 || prob->device != s->device) return VMC_ERR_INVALID_ARGUMENT;
+  RunModel model;
+  if (run_model(run, &model) != VMC_OK) return VMC_ERR_INVALID_ARGUMENT;
+  if (s->desc.tally == VMC_TALLY_FIXED64 && run->Nphoton >= kFixedMaxPhotons) return VMC_ERR_INVALID_ARGUMENT;
   std::unique_ptr<Request> r(new Request());
+  r->model = model;
   r->prob = prob;
   r->run = *run;
   r->run.precision = s->desc.precision;

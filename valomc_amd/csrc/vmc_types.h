@@ -136,7 +136,7 @@ struct WalkParams {
   void *etally;
   void *btally;
   double *ledger;               // [5] absorbed, exited, killed, gained, lost
-  unsigned long long *counters; // [4] photon dispenser, steps, loss, rechecks
+  unsigned long long *counters; // [8] photon dispenser, steps, loss, rechecks, fixed-point range flag
   unsigned long long *item_loss; // [nitems] photons lost per item (MC3D::loss)
   int64_t ne, nb;               // arena element slots (whole sectors) and boundary slots
   int32_t nrep;

@@ -109,6 +109,10 @@ public:
     rd.seed = (uint64_t)seed;
     rd.precision = precision;
     rd.tally = tally;
+    rd.model = 1;   // the members, as the reference's roulette and k read them
+    rd.weight0 = weight0;
+    rd.chance = chance;
+    rd.c0 = c0;
     vmc_output o = {ER.data, EI.data, EBR.data, EBI.data, DEBR.data, DEBI.data};
     Progress pg{progress, 0};
     const int st = vmc_problem_run(prob, &rd, &o, &stats, progress ? &Progress::call : nullptr, &pg);

@@ -40,6 +40,12 @@ class Problem:
     phase0: float = 0.0
     Nphoton: int = 1_000_000
     seed: int = 5489
+    # model constants, the reference's members MC3D::weight0, chance and c0
+    # (MC3D.hpp:146,160, defaults at :178-183): roulette threshold, roulette
+    # survival chance, speed of light in mm/s
+    weight0: float = 0.001
+    chance: float = 0.1
+    c0: float = 2.99792458e11
     notes: dict = field(default_factory=dict)
 
     @property
