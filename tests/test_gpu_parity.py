@@ -49,20 +49,23 @@ def _run(vmc, p, **kw):
 @pytest.mark.parametrize("name", sorted(cases.CASES))
 def test_fp64_matches_oracle_on_same_streams(vmc, name, tally):
     """FP64 transport against the Philox-mode oracle, photon for photon,
-    with FP64 tallies and with the bit-reproducible fixed-point tallies
-    (deposits rounded to 2^-32: far below the 1e-9 bound).  The roulette
-    cases run the reference's weight0/chance/c0 members at non-default
-    values."""
+    with FP64 tallies (relative L2 <= 1e-9) and with the bit-reproducible
+    fixed-point tallies, whose every flushed run of deposits is rounded to a
+    multiple of 2^-32 of a photon weight: a CW visit deposits ~1e-3 of a
+    weight, so an element's sum carries ~1e-8 / sqrt(runs) relative
+    rounding and the bound is 1e-7.  The roulette cases run the reference's
+    weight0/chance/c0 members at non-default values."""
     p = _golden(name)
     print(f"case {name} Ne={p.Ne} N={p.Nphoton} tally={tally}", flush=True)
     ref = pyoracle.run(p, 4, pyoracle.RNG_PHILOX)
     res = _run(vmc, p, precision=vmc.FP64, tally=vmc.TALLY_FIXED64 if tally == "fixed" else vmc.TALLY_FP64)
+    tol = 1e-7 if tally == "fixed" else 1e-9
     for k in OUT:
         a, b = getattr(res, k), getattr(ref, k)
         if np.linalg.norm(b) == 0:
             assert np.max(np.abs(a)) < 1e-12, k
         else:
-            assert rel_l2(a, b) < 1e-9, (k, rel_l2(a, b))
+            assert rel_l2(a, b) < tol, (k, rel_l2(a, b))
     assert abs(res.loss - ref.loss) <= max(2, p.Nphoton // 1000), (res.loss, ref.loss)
 
 
