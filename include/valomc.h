@@ -166,6 +166,25 @@ int vmc_problem_create(const vmc_problem_desc *desc, vmc_problem **out);
 int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
                     vmc_stats *stats, vmc_progress_fn progress, void *user);
 
+/* vmc_problem_run that allocates no device memory.  No counterpart in the
+   reference: for callers that cannot call hipMalloc (tight serving loops,
+   harnesses that forbid device allocation).  Same arguments, results and
+   status codes as vmc_problem_run; the tally replicas, counters, ledger and
+   launch record live in the library's static device scratch of 64 MiB per
+   GPU (held until the process ends) instead of the problem's own arena.
+   - The element records of run->precision must already be on the device:
+     the first vmc_problem_run (or batch) of that precision on the problem
+     allocates them; after that, new media are uploaded in place.
+     Otherwise VMC_ERR_INVALID_ARGUMENT.
+   - A run whose tallies do not fit the scratch (one replica per XCD is
+     reduced to a single one first) returns VMC_ERR_OUT_OF_MEMORY: up to
+     about 2M tetrahedra in CW or FD mode.
+   - Every call on a GPU shares that GPU's scratch and clears what it uses
+     at its start; overlapping calls on one GPU would race on it, so the
+     library serialises the calls of a process with a per-GPU lock. */
+int vmc_problem_run_noalloc(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
+                            vmc_stats *stats, vmc_progress_fn progress, void *user);
+
 /* Autotuner for a problem's launches (no counterpart in the reference).
    Times short runs of *run on prob -- trial_photons photons each, 0 = 2 %
    of run->Nphoton within [1e5, 1e7] -- over the launch shape (block_size,

@@ -110,6 +110,7 @@ def load_library():
         lib.vmc_problem_create.argtypes = [C.POINTER(ProblemDesc), C.POINTER(C.c_void_p)]
         lib.vmc_problem_run.argtypes = [C.c_void_p, C.POINTER(RunDesc), C.POINTER(Output),
                                         C.POINTER(Stats), C.c_void_p, C.c_void_p]
+        lib.vmc_problem_run_noalloc.argtypes = lib.vmc_problem_run.argtypes
         lib.vmc_problem_get_hn.argtypes = [C.c_void_p, C.c_void_p]
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
@@ -343,7 +344,8 @@ class Problem:
             lds_slots: int = 0, direct_flush: int = 0, chunk: int = 0, cache_epoch: int = 0,
             first_photon: int = 0, weight0: Optional[float] = None, chance: Optional[float] = None,
             c0: Optional[float] = None,
-            progress: Optional[Callable[[float], bool]] = None, check: bool = True) -> RunResult:
+            progress: Optional[Callable[[float], bool]] = None, check: bool = True,
+            allocate: bool = True) -> RunResult:
         """vmc_problem_run, MonteCarlo() of the reference (MC3D.hpp:1641-1822):
         transport ``nphoton`` photons (stream ids first_photon ..
         first_photon + nphoton - 1 of (seed, stream_tag)) and return the
@@ -356,7 +358,9 @@ class Problem:
         fixed-point results.  ``progress(percent)``
         returning False aborts (status STATUS_ABORTED, outputs normalised by
         the photons planned, as the reference does); ``check`` raises VmcError
-        on any other non-zero status."""
+        on any other non-zero status.  ``allocate=False`` calls
+        vmc_problem_run_noalloc (the library's static device scratch instead
+        of device allocations; needs an earlier run of the same precision)."""
         rd = make_run_desc(nphoton, seed=seed, f=f, phase0=phase0, precision=precision, tally=tally,
                            block_size=block_size, blocks_per_cu=blocks_per_cu, replicas=replicas,
                            refill_min=refill_min, stream_tag=stream_tag, lds_slots=lds_slots,
@@ -367,8 +371,8 @@ class Problem:
         cb = None
         if progress is not None:
             cb = PROGRESS_FN(lambda pct, user: 1 if progress(pct) else 0)
-        st = self.lib.vmc_problem_run(self._h, C.byref(rd), C.byref(o), C.byref(s),
-                                      C.cast(cb, C.c_void_p) if cb else None, None)
+        fn = self.lib.vmc_problem_run if allocate else self.lib.vmc_problem_run_noalloc
+        st = fn(self._h, C.byref(rd), C.byref(o), C.byref(s), C.cast(cb, C.c_void_p) if cb else None, None)
         if st and (check and st != STATUS_ABORTED):
             raise VmcError(st, self.lib)
         return RunResult(status=st, loss=int(s.loss), steps=int(s.steps), kernel_ms=float(s.kernel_ms),

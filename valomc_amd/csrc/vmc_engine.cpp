@@ -426,11 +426,67 @@ void finish_group(vmc_batch_item *items, const BatchGroup &g, const std::vector<
 
 
 // ---------------------------------------------------------------------------
-// One problem's MonteCarlo() (vmc_problem_run)
+// One problem's MonteCarlo() (vmc_problem_run, vmc_problem_run_noalloc)
 // ---------------------------------------------------------------------------
 
+namespace {
+// the scratch of a GPU is shared by every allocation-free run on it: the
+// runs of this process take turns
+std::mutex g_scratch_mu[kMaxDevices];
+} // namespace
+
+// Lays the arena of a run over ne element and nb boundary slots out in the
+// device's scratch (no allocation): the replicas, the reduced words in
+// place of replica 0 (the replica sums read every replica of a word before
+// writing it), ledger, counters, the item's loss count and its record.
+// One replica per XCD falls back to a single one when the copies do not
+// fit; the borrowed stream and events are the problem's.  A status.
+int scratch_arena(const vmc_problem &m, const vmc_run_desc *run, LaunchCfg &c, int64_t ne, int64_t nb, Arena &a) {
+  void *base = nullptr;
+  hip_check(scratch_address(m.device, &base), "hipGetSymbolAddress(scratch)");
+  auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  auto need = [&](int nrep) {
+    return up((size_t)ne * c.W * nrep * 8) + up((size_t)nb * 4 * nrep * 8) + 3 * 256 + up(sizeof(ItemRec));
+  };
+  if (need(c.nrep) > kScratchBytes && run->replicas <= 0 && c.nrep > 1) {
+    c.nrep = 1;
+    c.xcc = false;
+  }
+  if (need(c.nrep) > kScratchBytes) {
+    g_last_error = "run needs " + std::to_string(need(c.nrep)) + " bytes of tallies; the allocation-free scratch holds " +
+                   std::to_string(kScratchBytes);
+    return VMC_ERR_OUT_OF_MEMORY;
+  }
+  unsigned char *p = static_cast<unsigned char *>(base);
+  auto carve = [&](size_t bytes) {
+    unsigned char *q = p;
+    p += up(bytes);
+    return q;
+  };
+  a.etally.p = reinterpret_cast<unsigned long long *>(carve((size_t)ne * c.W * c.nrep * 8));
+  a.etally.n = (size_t)ne * c.W * c.nrep;
+  a.btally.p = reinterpret_cast<unsigned long long *>(carve((size_t)nb * 4 * c.nrep * 8));
+  a.btally.n = (size_t)nb * 4 * c.nrep;
+  a.raw_e.p = reinterpret_cast<double *>(a.etally.p);
+  a.raw_e.n = (size_t)ne * c.W;
+  a.raw_b.p = reinterpret_cast<double *>(a.btally.p);
+  a.raw_b.n = (size_t)nb * 4;
+  a.ledger.p = reinterpret_cast<double *>(carve(8 * 8));
+  a.ledger.n = 8;
+  a.counters.p = reinterpret_cast<unsigned long long *>(carve(kCounters * 8));
+  a.counters.n = kCounters;
+  a.item_loss.p = reinterpret_cast<unsigned long long *>(carve(8));
+  a.item_loss.n = 1;
+  a.items.p = reinterpret_cast<ItemRec *>(carve(sizeof(ItemRec)));
+  a.items.n = 1;
+  a.stream = m.arena.stream;
+  a.ev0 = m.arena.ev0;
+  a.ev1 = m.arena.ev1;
+  return VMC_OK;
+}
+
 int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
-                vmc_progress_fn progress, void *user) {
+                vmc_progress_fn progress, void *user, bool use_scratch) {
   if (!m || !run || run->Nphoton < 0) return VMC_ERR_INVALID_ARGUMENT;
   const auto t_start = std::chrono::steady_clock::now();
   const bool fd = 2.0 * M_PI * run->f > 0.0;
@@ -439,18 +495,38 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
   if (st != VMC_OK) return st;
   const int64_t N = run->Nphoton;
   if (c.fixed && N >= kFixedMaxPhotons) return VMC_ERR_INVALID_ARGUMENT;
+  if (use_scratch && (c.f64 ? m->med->elems64.p : m->med->elems32.p) == nullptr) {
+    // the records are allocated by the first run of a precision; refreshing
+    // them later (a new medium) reuses that allocation
+    g_last_error = "element records of this precision are not on the device yet: run vmc_problem_run once first";
+    return VMC_ERR_INVALID_ARGUMENT;
+  }
   int aborted = 0, grid = 0;
   float kernel_ms = 0.f;
   double led[5] = {0, 0, 0, 0, 0};
   unsigned long long cnt[kCounters] = {};
-  std::vector<double> raw_e, raw_b;
-  std::vector<unsigned long long> loss;
+  // host staging of the raw words: kept per thread on the allocation-free
+  // path, so repeated runs reuse it
+  std::vector<double> own_e, own_b;
+  std::vector<unsigned long long> own_loss;
+  thread_local std::vector<double> keep_e, keep_b;
+  thread_local std::vector<unsigned long long> keep_loss;
+  std::vector<double> &raw_e = use_scratch ? keep_e : own_e, &raw_b = use_scratch ? keep_b : own_b;
+  std::vector<unsigned long long> &loss = use_scratch ? keep_loss : own_loss;
   const int64_t ne = sector_round(m->Ne), nb = m->Nb;
   try {
     hip_check(hipSetDevice(m->device), "hipSetDevice");
     upload_params(*m, c.f64);
     choose_replicas(c, run, ne, nb);
-    Arena &a = m->arena;
+    Arena view;
+    std::unique_lock<std::mutex> turn;
+    if (use_scratch) {
+      if (m->device < 0 || m->device >= kMaxDevices) return VMC_ERR_INVALID_ARGUMENT;
+      turn = std::unique_lock<std::mutex>(g_scratch_mu[m->device]);
+      const int sst = scratch_arena(*m, run, c, ne, nb, view);
+      if (sst != VMC_OK) return sst;
+    }
+    Arena &a = use_scratch ? view : m->arena;
     arena_reset(a, c, ne, nb, 1);
     // photon ranges: one launch, or ~20 when a progress callback wants ticks
     const int nseg = progress ? 20 : 1;
@@ -681,7 +757,14 @@ int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) try {
 
 int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
                     vmc_progress_fn progress, void *user) try {
-  return problem_run(m, run, out, stats, progress, user);
+  return problem_run(m, run, out, stats, progress, user, false);
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
+}
+
+int vmc_problem_run_noalloc(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
+                            vmc_progress_fn progress, void *user) try {
+  return problem_run(m, run, out, stats, progress, user, true);
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
 }

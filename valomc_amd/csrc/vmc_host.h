@@ -30,6 +30,9 @@ hipError_t launch_fill_ranges(unsigned long long *p, size_t n, int nrep, size_t 
                               hipStream_t s);
 hipError_t launch_reduce(bool fixed, const void *rep, double *out, size_t n, int nrep, size_t stride,
                          double inv_scale, hipStream_t s);
+// device address of the allocation-free entry point's scratch (kScratchBytes)
+// on `device`, the current device
+hipError_t scratch_address(int device, void **ptr);
 
 
 // last error text of the calling thread (vmc_last_error)

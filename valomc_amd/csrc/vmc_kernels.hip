@@ -1159,10 +1159,45 @@ __global__ void reduce_replicas_kernel(const void *rep, double *out, size_t n, i
 } // namespace vmc
 
 // ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (vmc_problem_run_noalloc): the
+// tally replicas, counters, ledger and item record of one run live in this
+// static device array instead of library allocations.  Each GPU the library
+// runs on holds its own copy, at its own address, until the process ends;
+// every call on that GPU uses the same copy.
+// ---------------------------------------------------------------------------
+
+namespace vmc {
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+}  // namespace vmc
+
+#include <mutex>
+
+// ---------------------------------------------------------------------------
 // Host-side launchers (called from the launch engine, vmc_engine.cpp)
 // ---------------------------------------------------------------------------
 
 namespace vmc {
+
+// Device address of `scratch` on `device`, which must be the current device.
+// The name `scratch` is only a host-side stand-in: its address is looked up
+// once per device with hipGetSymbolAddress, and only that address is handed
+// to kernels and copies.
+hipError_t scratch_address(int device, void **ptr) {
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices || !ptr) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 typedef void (*WalkFn)(const WalkParams);
 

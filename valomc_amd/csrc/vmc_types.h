@@ -35,6 +35,10 @@ template <> struct Vec2T<double> { typedef d2 type; };
 constexpr int32_t kNbNone = INT32_MIN;
 constexpr int64_t kMaxElements = (int64_t)1 << 28;
 
+// Static device scratch of the allocation-free run entry point, per GPU
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+
 // Face f: outward unit normal n and offset d with n.x <= d inside.
 // opt = (mua, mus, n, g).  nbo[f] = (mus, n) of the element across face f
 // (zero on the boundary): a crossing needs them (free-path rescale and the
