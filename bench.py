@@ -13,7 +13,12 @@ Prints one JSON line.  `value` is photons/s over the timed steps with the
 mesh already resident in HBM.  `roofline` uses the walk kernel's duration
 from HIP events on its own stream; `cpu_baseline` times the reference
 engine (compiled from its sources into oracle/_ref/) with OpenMP on the host
-cores over a bounded photon sample of the same problem.
+cores over a bounded photon sample of the same problem (at least 30 s).
+The same line carries the metric's other two parts: `batch` (256 forward
+solves on the same box, 64 pencil positions x 4 optical-parameter sets, one
+autotuned persistent launch replayed from a HIP graph, FP32 against FP64)
+and `serve` (a Poisson stream of mixed 2D/3D, CW/100 MHz requests through
+the serving loop: photons/s, p50/p99 latency, alone vs in the mix).
 """
 from __future__ import annotations
 
@@ -34,11 +39,12 @@ HBM_PEAK_GBS = 8000.0
 # optical parameters of the element, plus the tally read-modify-write
 B_STEP = {("fp32", False): 112, ("fp32", True): 128, ("fp64", False): 192, ("fp64", True): 208}
 B_STEP_2D = 96   # 2D FP32 CW: 3 edge lines + neighbour codes + optics + tally
-# vector arithmetic per element step, an estimate: 3-4 plane tests ~36 FLOP,
-# move 6, deposit and weight ~6, scatter amortised ~6, one or two
-# transcendentals ~6
-FLOP_STEP = 60
+# vector arithmetic per element step when no counted profile is committed,
+# an estimate: 3-4 plane tests ~36 FLOP, move 6, deposit and weight ~6,
+# scatter amortised ~6, one or two transcendentals ~6
+FLOP_STEP_ESTIMATE = 60
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def parse():
@@ -48,8 +54,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=float, default=None,
                     help="photons per step (default: 1e8 box, 3e8 fd, 1e6 square2d)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="target duration of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=30.0,
+                    help="least duration of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--block-size", type=int, default=0)
@@ -58,6 +64,12 @@ def parse():
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
     ap.add_argument("--requests", type=int, default=1000, help="serve workload: number of requests")
+    ap.add_argument("--serve-requests", type=int, default=500,
+                    help="default line's serve part: number of requests (0: leave the part out)")
+    ap.add_argument("--serve-pool-scale", type=float, default=1.0,
+                    help="serve: mesh sizes of the serving pool times this (1.0: 1k-200k triangles, 50k-2M tets)")
+    ap.add_argument("--batch-scale", type=float, default=1.0,
+                    help="default line's batch part: photon counts times this (0: leave the part out)")
     ap.add_argument("--policy", choices=("fair", "shortest"), default="shortest",
                     help="serve workload: the server's scheduling policy")
     ap.add_argument("--utilization", type=float, default=0.8,
@@ -96,6 +108,20 @@ def copy_reference(nbytes: int):
 
 
 TRAFFIC_PROFILES = {"box": "walk3d_fp32_box384k_hbm_traffic.txt", "fd": "walk3d_fp32fd_box2m_hbm_traffic.txt"}
+FLOP_PROFILES = {"box": "walk3d_fp32_box384k_flops.txt", "fd": "walk3d_fp32fd_box2m_flops.txt"}
+
+
+def _profile_value(name, key):
+    """The number after `key:` in a committed profile summary (None if the
+    file or the line is absent)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fh:
+            for line in fh:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def traffic_per_step(workload="box"):
@@ -103,15 +129,16 @@ def traffic_per_step(workload="box"):
     committed FETCH_SIZE / WRITE_SIZE profile (None if absent)."""
     if workload not in TRAFFIC_PROFILES:
         return None
-    path = os.path.join(ROOT, "profiles", TRAFFIC_PROFILES[workload])
-    try:
-        with open(path) as fh:
-            for line in fh:
-                if line.startswith("hbm_bytes_per_element_step:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+    return _profile_value(TRAFFIC_PROFILES[workload], "hbm_bytes_per_element_step")
+
+
+def flops_per_step(workload="box"):
+    """Counted FP32 vector FLOPs per element step of the workload's FP32
+    walk (SQ_INSTS_VALU_FLOPS_FP32 over the steps of the same dispatch),
+    from its committed profile; None if absent."""
+    if workload not in FLOP_PROFILES:
+        return None
+    return _profile_value(FLOP_PROFILES[workload], "fp32_flop_per_element_step")
 
 
 def cpu_model() -> str:
@@ -126,7 +153,8 @@ def cpu_model() -> str:
 
 
 def cpu_baseline(p, target_s: float, what: str = "the same problem"):
-    """Time the reference engine (or, without it, the oracle) on the host."""
+    """Time the reference engine (or, without it, the oracle) on the host:
+    a pilot of at least 1 s sizes a sample of at least `target_s`."""
     # the OpenMP runtime reads these when the reference library first loads
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
@@ -135,11 +163,14 @@ def cpu_baseline(p, target_s: float, what: str = "the same problem"):
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     kind = "reference" if pyoracle.have_reference() else "port"
     run = pyoracle.run_reference if kind == "reference" else pyoracle.run
-    # calibrate on a short run, then size the sample to ~target_s
     p.Nphoton = 20000
-    r = run(p, threads)
+    while True:
+        r = run(p, threads)
+        if r.seconds >= 1.0 or p.Nphoton >= 10_000_000:
+            break
+        p.Nphoton = int(p.Nphoton * min(20.0, 1.5 / max(r.seconds, 1e-3)))
     rate = p.Nphoton / max(r.seconds, 1e-6)
-    n = int(min(max(rate * target_s, 20000), 5e7))
+    n = int(min(max(rate * target_s * 1.1, 20000), 2e8))
     p.Nphoton = n
     r = run(p, threads)
     return {"value": n / r.seconds, "unit": "photons/s", "cores": threads, "kind": kind,
@@ -148,16 +179,18 @@ def cpu_baseline(p, target_s: float, what: str = "the same problem"):
                       f"OMP_PLACES={os.environ.get('OMP_PLACES')}) on {cpu_model()}, {r.seconds:.1f} s"}
 
 
-def run_batch(a):
+def batch_leg(a, photon_scale: float, steps: int, full: bool):
     """256 forward solves on the 384k-tet box (64 pencil positions x 4
     optical-parameter sets, FP32 transport, separate FP64 tallies per
     instance): launch knobs picked by vmc_batch_autotune on the batch
     itself, the batch recorded once as a HIP graph (vmc_batch_plan) and
-    replayed -- the timed figure -- next to the same batch launched eagerly
-    and the same items run one vmc_problem_run at a time."""
+    replayed `steps` times -- the timed figure -- with FP32 against FP64
+    runs of the same streams for the first four items.  `full` adds the
+    same batch launched eagerly and the items run one vmc_problem_run at a
+    time.  Returns the figures."""
     import valomc_amd as vmc
     from valomc_amd import meshgen
-    base_p, sources, media, photons = meshgen.batched_box_instances(seed=1234, photon_scale=a.photon_scale)
+    base_p, sources, media, photons = meshgen.batched_box_instances(seed=1234, photon_scale=photon_scale)
     base = vmc.Problem(base_p)
     meds = [base.variant(mua=mua, mus=mus) for mua, mus in media]
     probs = [[m.variant(BCType=s["BCType"], BCLDirType=s["BCLDirType"], BCLNormal=s["BCLNormal"])
@@ -168,85 +201,104 @@ def run_batch(a):
     kw = dict(precision=vmc.FP32, block_size=a.block_size, refill_min=a.refill_min, replicas=a.replicas,
               tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
     small = [(pr, dict(nphoton=max(1, k["nphoton"] // 100), seed=k["seed"])) for pr, k in items]
-    vmc.batch_run(small, **kw)   # warm-up
+    vmc.batch_run(small, **kw)   # warm-up: code loading, arena allocation
     print(f"batch: {len(items)} items, {total:.3e} photons; autotune", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    tuned = vmc.batch_autotune(items, **kw)
+    tuned = vmc.batch_autotune(items, trial_fraction=0.02 if full else 0.005, **kw)
     tuned["tuning_s"] = round(time.perf_counter() - t0, 3)
     kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots", "cache_epoch")})
-    # the batch as a HIP graph: recorded once, replayed (a warm-up replay
-    # with other seeds, then the timed replays)
+    # the batch as a HIP graph: recorded once, replayed (with a full run, a
+    # warm-up replay with other seeds first)
     t0 = time.perf_counter()
     plan = vmc.BatchPlan(items, **kw)
     plan_s = time.perf_counter() - t0
-    print(f"batch: knobs {tuned}; graph warm-up replay", file=sys.stderr, flush=True)
-    plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
+    print(f"batch: knobs {tuned}; graph replays", file=sys.stderr, flush=True)
+    if full:
+        plan.run([(pr, dict(k, seed=k["seed"] + 7)) for pr, k in items])
     walls = []
-    for i in range(a.steps):
+    for i in range(steps):
         t0 = time.perf_counter()
         res, st = plan.run()
         walls.append(time.perf_counter() - t0)
-        print(f"batch: replay {i + 1}/{a.steps} {walls[-1]:.2f} s", file=sys.stderr, flush=True)
+        print(f"batch: replay {i + 1}/{steps} {walls[-1]:.2f} s", file=sys.stderr, flush=True)
     wall = sum(walls) / len(walls)
     plan.close()
-    # the same batch launched eagerly
-    t0 = time.perf_counter()
-    eres, est = vmc.batch_run(items, **kw)
-    eager_wall = time.perf_counter() - t0
-    print(f"batch: eager launch {eager_wall:.2f} s; one solve at a time", file=sys.stderr, flush=True)
-    graph_same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, eres))
-    # the same items one solve at a time
-    t0 = time.perf_counter()
-    seq_kernel = 0.0
-    seq = []
-    for pr, k in items:
-        r = pr.run(k["nphoton"], seed=k["seed"], **kw)
-        seq_kernel += r.kernel_ms
-        seq.append(r)
-    seq_wall = time.perf_counter() - t0
-    print(f"batch: sequential {seq_wall:.2f} s; FP64 runs of the first 4 items", file=sys.stderr, flush=True)
-    same = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER)) for r1, r2 in zip(res, seq))
+    fig = {"instances": len(items), "photons": total, "photon_scale": photon_scale,
+           "photons_per_s": round(total / wall, 1), "ms": round(wall * 1e3, 3),
+           "kernel_ms": round(st.kernel_ms, 3),
+           "element_steps_per_s": round(st.steps / (st.kernel_ms / 1e3), 1),
+           "autotune": tuned, "graph_plan_create_s": round(plan_s, 3)}
+    if full:
+        # the same batch launched eagerly
+        t0 = time.perf_counter()
+        eres, est = vmc.batch_run(items, **kw)
+        eager_wall = time.perf_counter() - t0
+        print(f"batch: eager launch {eager_wall:.2f} s; one solve at a time", file=sys.stderr, flush=True)
+        fig["eager_batch_photons_per_s"] = round(total / eager_wall, 1)
+        fig["graph_vs_eager_rel_l2_max"] = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER))
+                                               for r1, r2 in zip(res, eres))
+        # the same items one solve at a time
+        t0 = time.perf_counter()
+        seq_kernel = 0.0
+        seq = []
+        for pr, k in items:
+            r = pr.run(k["nphoton"], seed=k["seed"], **kw)
+            seq_kernel += r.kernel_ms
+            seq.append(r)
+        seq_wall = time.perf_counter() - t0
+        fig["sequential_photons_per_s"] = round(total / seq_wall, 1)
+        fig["sequential_kernel_ms"] = round(seq_kernel, 3)
+        fig["batch_vs_sequential_rel_l2_max"] = max(float(np.linalg.norm(r1.ER - r2.ER) / np.linalg.norm(r2.ER))
+                                                    for r1, r2 in zip(res, seq))
+    print("batch: FP64 runs of the first 4 items", file=sys.stderr, flush=True)
     # accuracy: FP32 batch items against FP64 runs of the same streams (first 4)
     rel = []
     for (pr, k), r in list(zip(items, res))[:4]:
         r64 = pr.run(k["nphoton"], seed=k["seed"], precision=vmc.FP64)
         rel.append(float(np.linalg.norm(r.ER - r64.ER) / np.linalg.norm(r64.ER)))
+    fig["fp32_vs_fp64_rel_l2_first4"] = rel
+    for row in probs:
+        for pr in row:
+            pr.close()
+    for m in meds:
+        m.close()
+    base.close()
+    return fig
+
+
+def run_batch(a):
+    """The batch workload as its own line (`--workload batch`)."""
+    fig = batch_leg(a, a.photon_scale, a.steps, full=True)
     line = {
-        "metric": METRIC, "value": round(total / wall, 1), "unit": "photons/s", "n_gpus": 1,
-        "steps": a.steps, "warmup": 1, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
+        "metric": METRIC, "value": fig["photons_per_s"], "unit": "photons/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": 1, "ms_per_step": fig["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "256 instances on the 384k-tet box (64 pencil positions x 4 optical-parameter "
                                "sets), one persistent launch replayed from a HIP graph, autotuned, FP32 "
                                "transport, separate FP64 tallies",
-                   "instances": len(items), "photons_per_step": total, "photon_scale": a.photon_scale,
-                   "parallelism": "single GPU"},
-        "autotune": tuned,
-        "graph_plan_create_s": round(plan_s, 3),
-        "kernel_ms_per_step": round(st.kernel_ms, 3),
-        "element_steps_per_s": round(st.steps / (st.kernel_ms / 1e3), 1),
-        "eager_batch_photons_per_s": round(total / eager_wall, 1),
-        "graph_vs_eager_rel_l2_max": graph_same,
-        "sequential_photons_per_s": round(total / seq_wall, 1),
-        "sequential_kernel_ms": round(seq_kernel, 3),
-        "batch_vs_sequential_rel_l2_max": same,
-        "fp32_vs_fp64_rel_l2_first4": rel,
+                   "instances": fig["instances"], "photons_per_step": fig["photons"],
+                   "photon_scale": a.photon_scale, "parallelism": "single GPU"},
+        **{k: v for k, v in fig.items() if k not in ("instances", "photons", "photon_scale", "photons_per_s", "ms")},
     }
     print(json.dumps(line), flush=True)
 
 
-def run_serve(a):
-    """Poisson stream of forward solves through the serving loop: 40% 2D
-    meshes of 1k-200k triangles and 60% 3D boxes of 50k-2M tetrahedra from a
-    seeded pool of structured-then-jittered meshes kept resident, photons
-    lognormal (median 1e7, sigma 1), CW or 100 MHz half and half.  The rate
-    gives about `utilization` of the service rate measured per mesh first."""
+def serve_leg(a, nreq: int, ncal: int):
+    """Poisson stream of `nreq` forward solves through the serving loop: 40%
+    2D meshes of 1k-200k triangles and 60% 3D boxes of 50k-2M tetrahedra
+    from a seeded pool of structured-then-jittered meshes kept resident,
+    photons lognormal (median 1e7, sigma 1), CW or 100 MHz half and half.
+    The rate gives about `utilization` of the service rate measured per mesh
+    first, corrected on a calibration stream of `ncal` requests.  Returns
+    the figures."""
     import valomc_amd as vmc
     from valomc_amd import meshgen, serving
-    pool = meshgen.serving_pool(seed=2024)
-    reqs = meshgen.serving_requests(a.requests, seed=2025)
+    pool = meshgen.serving_pool(seed=2024, scale=a.serve_pool_scale)
+    reqs = meshgen.serving_requests(nreq, seed=2025)
     cache = serving.MeshCache(budget_bytes=16 << 30)
     kw = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
     policy = vmc.SERVE_SHORTEST if a.policy == "shortest" else vmc.SERVE_FAIR
+    print(f"serve: service rates of the {len(pool)} meshes", file=sys.stderr, flush=True)
     # service rate per (mesh, mode) from short single runs
     rate = {}
     for k, p in enumerate(pool):
@@ -261,7 +313,7 @@ def run_serve(a):
     # requests in flight together share launches, so the device is busy for
     # less than the sum of their alone times: one correction of the rate
     # from a calibration stream (other seeds) at the estimated rate
-    ncal = min(len(reqs), 150)
+    ncal = min(len(reqs), ncal)
     cal = [dict(q, seed=q["seed"] + 100000) for q in reqs[:ncal]]
     print(f"serve: calibration stream of {ncal} requests at {lam0:.2f}/s", file=sys.stderr, flush=True)
     with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
@@ -273,8 +325,8 @@ def run_serve(a):
     share = float(np.sum(service[:ncal])) / max(busy0 * wcal, 1e-9)
     lam = lam0 * min(2.0, max(0.5, share))
     keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
-    print(f"serve: calibration busy {busy0:.3f}, sharing {share:.3f}; {len(reqs)} requests at {lam:.2f}/s", file=sys.stderr,
-          flush=True)
+    print(f"serve: calibration busy {busy0:.3f}, sharing {share:.3f}; {len(reqs)} requests at {lam:.2f}/s",
+          file=sys.stderr, flush=True)
     with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
         out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
         ctr = srv.counters()
@@ -291,22 +343,33 @@ def run_serve(a):
         cache.release(prob)
         diffs.append(float(np.linalg.norm(alone.ER - o["ER"]) / np.linalg.norm(alone.ER)))
     cache.close()
-    line = {
-        "metric": METRIC, "value": round(total / wall, 1), "unit": "photons/s", "n_gpus": 1,
-        "steps": len(out), "warmup": 0, "ms_per_step": round(wall / len(out) * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "serving: Poisson stream, 40% 2D (1k-200k triangles) / 60% 3D (50k-2M tets) "
-                               "resident jittered meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50",
-                   "requests": len(out), "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
-                   "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3),
-                                   "sharing_factor": round(share, 3)},
-                   "photons_total": total, "policy": a.policy, "parallelism": "single GPU"},
+    return {
+        "requests": len(out), "photons": total, "photons_per_s": round(total / wall, 1),
         "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
         "latency_ms_max": round(float(lat.max()), 2),
-        "wall_s": round(wall, 3), "gpu_busy_fraction": round(ctr["busy_ms"] / 1e3 / wall, 3),
-        "launches": ctr["launches"],
-        "alone_vs_mix_rel_l2_max": max(diffs) if diffs else None,
+        "arrival_rate_per_s": round(lam, 3), "target_utilization": a.utilization,
+        "calibration": {"requests": ncal, "rate_per_s": round(lam0, 3), "busy_fraction": round(busy0, 3),
+                        "sharing_factor": round(share, 3)},
+        "policy": a.policy, "wall_s": round(wall, 3), "gpu_busy_fraction": round(ctr["busy_ms"] / 1e3 / wall, 3),
+        "launches": ctr["launches"], "alone_vs_mix_rel_l2_max": max(diffs) if diffs else None,
+    }
+
+
+SERVE_DESC = ("serving: Poisson stream, 40% 2D (1k-200k triangles) / 60% 3D (50k-2M tets) resident jittered "
+              "meshes, photons lognormal(median 1e7, sigma 1), CW/100 MHz 50/50, FP32 transport")
+
+
+def run_serve(a):
+    """The serving workload as its own line (`--workload serve`)."""
+    fig = serve_leg(a, a.requests, 150)
+    line = {
+        "metric": METRIC, "value": fig["photons_per_s"], "unit": "photons/s", "n_gpus": 1,
+        "steps": fig["requests"], "warmup": 0, "ms_per_step": round(fig["wall_s"] / fig["requests"] * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": SERVE_DESC, "requests": fig["requests"], "photons_total": fig["photons"],
+                   "parallelism": "single GPU"},
+        **{k: v for k, v in fig.items() if k not in ("requests", "photons", "photons_per_s")},
     }
     print(json.dumps(line), flush=True)
 
@@ -380,8 +443,12 @@ def main():
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
 
-    vector = {"achieved": round(steps_total * FLOP_STEP / kernel_s / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
-              "unit": "TFLOP/s", "flop_per_step": FLOP_STEP}
+    fps = flops_per_step(a.workload) if p.dim == 3 else None
+    flop_step = fps if fps is not None else FLOP_STEP_ESTIMATE
+    vector = {"achieved": round(steps_total * flop_step / kernel_s / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+              "unit": "TFLOP/s", "flop_per_step": round(flop_step, 2),
+              "flop_source": "SQ_INSTS_VALU_FLOPS_FP32 per element step (committed profile)" if fps is not None
+              else "estimate"}
     vector["frac"] = round(vector["achieved"] / FP32_PEAK_TFLOPS, 4)
 
     extra = {}
@@ -403,7 +470,14 @@ def main():
     balance = abs(led[0] + led[1] + led[2] - led[3] + led[4] - nphot) / nphot
     cpu = None
     if not a.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(make(1), a.cpu_seconds, desc.split(", ")[0])
+    # the metric's batched and serving parts ride on the headline line
+    parts = {}
+    if a.workload == "box" and a.batch_scale > 0:
+        parts["batch"] = batch_leg(a, a.batch_scale, 1, full=False)
+    if a.workload == "box" and a.serve_requests > 0:
+        parts["serve"] = {"workload": SERVE_DESC, **serve_leg(a, a.serve_requests, 100)}
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
@@ -422,6 +496,7 @@ def main():
         "grid": [runs[-1].grid_blocks, runs[-1].block_size],
         "problem_create_s": round(t_create, 3),
         **extra,
+        **parts,
     }
     if tuned is not None:
         line["autotune"] = tuned

@@ -15,10 +15,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_bench_line_contract():
-    print("bench.py --photons 2e6 --steps 2 --warmup 1 --cpu-seconds 1", flush=True)
-    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--photons", "2e6", "--steps", "2",
-                          "--warmup", "1", "--cpu-seconds", "1"], capture_output=True, text=True, timeout=240,
-                         cwd=ROOT)
+    args = ["--photons", "2e6", "--steps", "2", "--warmup", "1", "--cpu-seconds", "1", "--batch-scale", "0.002",
+            "--serve-requests", "12", "--serve-pool-scale", "0.05"]
+    print("bench.py " + " ".join(args), flush=True)
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                         timeout=240, cwd=ROOT)
     assert out.returncode == 0, out.stderr[-2000:]
     lines = [ln for ln in out.stdout.splitlines() if ln.startswith("{")]
     assert len(lines) == 1, out.stdout
@@ -46,4 +47,14 @@ def test_bench_line_contract():
     assert c["kind"] in ("reference", "port") and c["sample"]
     # the FP32 line keeps within the metric's 1 % of FP64 on the same streams
     assert d["fp32_vs_fp64_rel_l2"] < 1e-2
+    # the metric's batched part: 256 instances in one replayed launch
+    bt = d["batch"]
+    assert bt["instances"] == 256 and bt["photons_per_s"] > 0 and bt["photons"] > 0
+    assert bt["photons_per_s"] == pytest.approx(bt["photons"] / (bt["ms"] / 1e3), rel=1e-3)
+    assert max(bt["fp32_vs_fp64_rel_l2_first4"]) < 1e-2
+    # and its serving part: latencies and alone == in the mix
+    sv = d["serve"]
+    assert sv["requests"] == 12 and sv["photons_per_s"] > 0
+    assert 0 < sv["latency_ms_p50"] <= sv["latency_ms_p99"] <= sv["latency_ms_max"]
+    assert sv["alone_vs_mix_rel_l2_max"] is not None and sv["alone_vs_mix_rel_l2_max"] < 1e-9
 
