@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--refill-min", type=int, default=0)
     ap.add_argument("--replicas", type=int, default=0)
     ap.add_argument("--tally", choices=("fp64", "fixed"), default="fp64")
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32",
+                    help="transport arithmetic of the timed steps (fp64: the plain FP64 path, for profiling it; "
+                         "the headline is fp32)")
     ap.add_argument("--requests", type=int, default=1000, help="serve workload: number of requests")
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="default line's serve part: number of requests (0: leave the part out)")
@@ -84,8 +87,11 @@ def parse():
                     help="box/fd/square2d: pick block size, refill threshold, photons per wave fetch and "
                          "tally strategy with vmc_problem_autotune before the timed steps (tuning time "
                          "is reported, not timed)")
+    ap.add_argument("--line-probe", action="store_true",
+                    help="only run valomc_amd/bin/line_probe: random 128-byte line reads and 64-byte atomic line "
+                         "requests per second at footprints from 2 MiB to 1 GiB, on the walk's launch shape")
     ap.add_argument("--copy-bytes", type=float, default=0,
-                    help="only run a streaming copy kernel (out = -in, float32) over this many bytes "
+                    help="only run a streaming kernel (out = 2 * in, float32) over this many bytes "
                          "three times: the reference dispatch that scales FETCH_SIZE / WRITE_SIZE")
     a = ap.parse_args()
     a.photons_given = a.photons is not None
@@ -95,16 +101,20 @@ def parse():
 
 
 def copy_reference(nbytes: int):
+    """A streaming kernel of known size: out = 2 * in over float32, three
+    launches.  Each launch reads and writes `nbytes` and makes one FP32
+    multiply per element, which scales FETCH_SIZE / WRITE_SIZE and shows how
+    SQ_INSTS_VALU_FLOPS_FP32 counts."""
     import torch
     n = nbytes // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     for _ in range(3):
-        torch.neg(src, out=dst)
+        torch.mul(src, 2.0, out=dst)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_kernel": "torch.neg float32", "bytes_read": 4 * n, "bytes_written": 4 * n,
-                      "launches": 3}), flush=True)
+    print(json.dumps({"copy_kernel": "torch.mul float32 by 2", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "fp32_flops": n, "launches": 3}), flush=True)
 
 
 TRAFFIC_PROFILES = {"box": "walk3d_fp32_box384k_hbm_traffic.txt", "fd": "walk3d_fp32fd_box2m_hbm_traffic.txt"}
@@ -383,6 +393,12 @@ def main():
     if a.copy_bytes:
         copy_reference(int(a.copy_bytes))
         return
+    if a.line_probe:
+        import subprocess
+        exe = os.path.join(ROOT, "valomc_amd", "bin", "line_probe")
+        if not os.path.exists(exe):
+            sys.exit(f"{exe} not built: run `python -m valomc_amd.build probe`")
+        sys.exit(subprocess.call([exe]))
     if a.workload == "batch":
         run_batch(a)
         return
@@ -406,6 +422,8 @@ def main():
         nphot = int(a.photons)
         make = lambda n: meshgen.sphere_box_3d(nphoton=n, seed=1234)
         desc = "384k-tet heterogeneous box, pencil source, CW, FP32 transport + FP64 tallies"
+    if a.precision == "fp64":
+        desc = desc.replace("FP32 transport + FP64 tallies", "FP64 transport + FP64 tallies")
     p = make(nphot)
     fd = float(getattr(p, "f", 0.0) or 0.0) > 0.0
     t_create = time.perf_counter()
@@ -421,38 +439,41 @@ def main():
         tuned = prob.autotune(nphot, precision=vmc.FP32, **kw)
         tuned["tuning_s"] = round(time.perf_counter() - t, 3)
         kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots", "cache_epoch")})
+    prec = vmc.FP64 if a.precision == "fp64" else vmc.FP32
     for i in range(a.warmup):
-        prob.run(nphot, seed=100 + i, precision=vmc.FP32, **kw)
+        prob.run(nphot, seed=100 + i, precision=prec, **kw)
     runs = []
     t0 = time.perf_counter()
     for i in range(a.steps):
-        runs.append(prob.run(nphot, seed=1 + i, precision=vmc.FP32, **kw))
+        runs.append(prob.run(nphot, seed=1 + i, precision=prec, **kw))
     t1 = time.perf_counter()
     wall = t1 - t0
     value = a.steps * nphot / wall
     kernel_s = sum(r.kernel_ms for r in runs) / 1e3
     steps_total = sum(r.steps for r in runs)
-    bstep = B_STEP[("fp32", fd)] if p.dim == 3 else B_STEP_2D
+    bstep = B_STEP[(a.precision, fd)] if p.dim == 3 else B_STEP_2D
     achieved = steps_total * bstep / kernel_s / 1e9
-    tps = traffic_per_step(a.workload)
+    tps = traffic_per_step(a.workload) if a.precision == "fp32" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if tps is None else round(tps * steps_total / a.steps),
-                "kernel": f"walk_kernel<{p.dim},float,{'FD' if fd else 'CW'},fp64-tally>",
+                "kernel": f"walk_kernel<{p.dim},{'double' if a.precision == 'fp64' else 'float'},"
+                          f"{'FD' if fd else 'CW'},{'fixed' if a.tally == 'fixed' else 'fp64'}-tally>",
                 "bytes_per_step": bstep,
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
 
-    fps = flops_per_step(a.workload) if p.dim == 3 else None
+    fps = flops_per_step(a.workload) if (p.dim == 3 and a.precision == "fp32") else None
     flop_step = fps if fps is not None else FLOP_STEP_ESTIMATE
-    vector = {"achieved": round(steps_total * flop_step / kernel_s / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+    vpeak = FP64_PEAK_TFLOPS if a.precision == "fp64" else FP32_PEAK_TFLOPS
+    vector = {"achieved": round(steps_total * flop_step / kernel_s / 1e12, 3), "peak": vpeak,
               "unit": "TFLOP/s", "flop_per_step": round(flop_step, 2),
               "flop_source": "SQ_INSTS_VALU_FLOPS_FP32 per element step (committed profile)" if fps is not None
               else "estimate"}
-    vector["frac"] = round(vector["achieved"] / FP32_PEAK_TFLOPS, 4)
+    vector["frac"] = round(vector["achieved"] / vpeak, 4)
 
     extra = {}
-    if not a.no_fp64:
+    if not a.no_fp64 and a.precision == "fp32":
         f32 = runs[0]
         t = time.perf_counter()
         f64 = prob.run(nphot, seed=1, precision=vmc.FP64, **kw)
@@ -482,8 +503,8 @@ def main():
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "photons/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(wall / a.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.precision == "fp64" else "f32", "data": "synthetic",
         "config": {"workload": desc, "elements": p.Ne, "boundary_faces": p.Nb, "nodes": p.Nr,
                    "photons_per_step": nphot, "parallelism": "single GPU"},
         "roofline": roofline,

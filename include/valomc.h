@@ -92,7 +92,8 @@ typedef struct vmc_run_desc {
                             0 = automatic: one per XCD (8, workgroup-scope
                             atomics), or a
                             single one when a copy is 4..64 MiB             */
-  int32_t refill_min;    /* dead lanes a wave collects before refilling, 0 = 6 */
+  int32_t refill_min;    /* dead lanes a wave collects before refilling,
+                            0 = default (10 CW, 16 frequency domain)        */
   uint32_t stream_tag;   /* extra stream word: runs with different tags are
                             independent (batch instances use their index)      */
   int32_t lds_slots;     /* per-workgroup LDS tally cache entries (sectors of

@@ -2,8 +2,9 @@
 
     python -m valomc_amd.build
 
-Outputs valomc_amd/libvalomc.so (the product) and valomc_amd/bin/MC2D,
-MC3D (the drop-in command-line executables).  ``build_oracle`` compiles the
+Outputs valomc_amd/libvalomc.so (the product), valomc_amd/bin/MC2D, MC3D
+(the drop-in command-line executables) and valomc_amd/bin/line_probe (the
+memory-ceiling probe bench.py runs).  ``build_oracle`` compiles the
 test oracle (and, where the reference tree exists, the reference engine)
 through oracle/Makefile.
 """
@@ -75,6 +76,13 @@ def build_cli() -> None:
               f"-Wl,-rpath,{HERE}"])
 
 
+def build_probe() -> None:
+    """The random-line probe (valomc_amd/bin/line_probe): memory ceilings
+    for the walk's access pattern, run by `bench.py --line-probe`."""
+    os.makedirs(BIN, exist_ok=True)
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "line_probe.hip"), "-o", os.path.join(BIN, "line_probe")])
+
+
 def build_oracle() -> None:
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-j8"])
 
@@ -82,10 +90,11 @@ def build_oracle() -> None:
 def build_all() -> None:
     build_library()
     build_cli()
+    build_probe()
     build_oracle()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_library, "cli": build_cli, "oracle": build_oracle}[w]()
+        {"all": build_all, "lib": build_library, "cli": build_cli, "probe": build_probe, "oracle": build_oracle}[w]()

@@ -80,8 +80,8 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   if (c->slots > (1 << 20)) return VMC_ERR_INVALID_ARGUMENT;
   c->lds = WalkLds(c->W, rbytes, c->slots, c->block).total;
   // creating photons for several dead lanes at once keeps the creation
-  // code's lanes busy; 6 was best on the 384k-tet box (4..16 within 1%)
-  c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : 6;
+  // code's lanes busy
+  c->refill = run->refill_min > 0 ? std::min(run->refill_min, 64) : default_refill(fd);
   // photon ids per dispenser fetch: a power of 2 (chunks never straddle
   // items), at least a wave's 64 lanes (one fetch refills all dead lanes)
   if (run->chunk != 0 && (run->chunk < 64 || run->chunk > 4096 || (run->chunk & (run->chunk - 1))))
@@ -305,7 +305,7 @@ int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trial
   const bool f64 = base.precision == VMC_FP64;
   // defaults spelled out, so a knob's default value is not timed twice
   if (best.block_size == 0) best.block_size = f64 ? 256 : 1024;
-  if (best.refill_min == 0) best.refill_min = 6;
+  if (best.refill_min == 0) best.refill_min = default_refill(2.0 * M_PI * base.f > 0.0);
   if (best.chunk == 0) best.chunk = (int32_t)kChunk;
   if (!std::isfinite(timed(best))) return VMC_ERR_INVALID_ARGUMENT;
   double tb = timed(best);
@@ -315,7 +315,7 @@ int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trial
   };
   const std::vector<Knob> knobs = {
       {&vmc_run_desc::block_size, f64 ? std::vector<int32_t>{256, 128, 64} : std::vector<int32_t>{1024, 512, 256}},
-      {&vmc_run_desc::refill_min, {6, 2, 16, 32}},
+      {&vmc_run_desc::refill_min, {6, 10, 16, 24, 32}},
       {&vmc_run_desc::chunk, {128, 64, 256, 512}},
       {&vmc_run_desc::replicas, {0, 8, 1}},
       {&vmc_run_desc::lds_slots, {0, -1}},
