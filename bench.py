@@ -142,6 +142,45 @@ def traffic_per_step(workload="box"):
     return _profile_value(TRAFFIC_PROFILES[workload], "hbm_bytes_per_element_step")
 
 
+COUNTER_PROFILES = {("box", "fp32"): "walk3d_fp32_box384k_counters.txt", ("fd", "fp32"): "walk3d_fp32fd_box2m_counters.txt",
+                    ("box", "fp64"): "walk3d_fp64_box384k_counters.txt"}
+# random 128-byte line reads and 64-byte FP64 atomic line requests per
+# second on the walk's launch shape (profiles/line_probe_mi355x.txt): reads
+# at the records' footprint (64 MiB: the box's FP32 records; 1 GiB: larger
+# ones), atomics at any footprint up to 192 MiB
+READ_CEILING = {("box", "fp32"): 61.44e9, ("box", "fp64"): 53.49e9, ("fd", "fp32"): 53.49e9}
+ATOMIC_CEILING = 23.5e9
+
+
+def memory_side(workload, precision, steps_per_s):
+    """The walk's memory-side request rates (TCC_EA0 read and atomic
+    requests per element step from its committed counter profile, times the
+    measured step rate) against the probe's ceilings; None without a
+    profile."""
+    key = (workload, precision)
+    if key not in COUNTER_PROFILES:
+        return None
+    name = COUNTER_PROFILES[key]
+    rd, at = None, None
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fh:
+            for ln in fh:
+                f = ln.split()
+                if len(f) >= 3 and f[0] == "TCC_EA0_RDREQ_sum":
+                    rd = float(f[2])
+                if len(f) >= 3 and f[0] == "TCC_EA0_ATOMIC_sum":
+                    at = float(f[2])
+    except (OSError, ValueError):
+        return None
+    if rd is None or at is None:
+        return None
+    return {"read_requests_per_step": rd, "atomic_requests_per_step": at,
+            "read_g_per_s": round(rd * steps_per_s / 1e9, 2), "atomic_g_per_s": round(at * steps_per_s / 1e9, 2),
+            "read_frac_of_probe": round(rd * steps_per_s / READ_CEILING[key], 3),
+            "atomic_frac_of_probe": round(at * steps_per_s / ATOMIC_CEILING, 3),
+            "source": f"profiles/{name}, profiles/line_probe_mi355x.txt"}
+
+
 def flops_per_step(workload="box"):
     """Counted FP32 vector FLOPs per element step of the workload's FP32
     walk (SQ_INSTS_VALU_FLOPS_FP32 over the steps of the same dispatch),
@@ -511,6 +550,7 @@ def main():
         "vector_roofline": vector,
         "cpu_baseline": cpu,
         "element_steps_per_s": round(steps_total / kernel_s, 1),
+        "memory_side": memory_side(a.workload, a.precision, steps_total / kernel_s) if p.dim == 3 else None,
         "energy_balance_rel": balance,
         "loss_per_step": runs[-1].loss,
         "rechecks_per_step": runs[-1].rechecks,
