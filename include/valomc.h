@@ -207,6 +207,16 @@ int vmc_problem_autotune(vmc_problem *prob, const vmc_run_desc *run, vmc_run_des
    light-source faces. */
 int vmc_problem_check(const vmc_problem_desc *desc, int64_t *HN_out, int64_t *nsources);
 
+/* Host-only light-source list of Init() (MC3D::BuildLightSource,
+   MC3D.hpp:941-1002; MC2D.hpp:851-911): the boundary faces that emit
+   light in the reference's element-major order, the element each belongs
+   to and the normalised cumulative distribution (face area, 2D edge
+   length, times BCIntensity) photons are drawn from.  Writes up to
+   `capacity` entries to each non-NULL array and the number of light-source
+   faces to *nsources (so a first call with capacity 0 sizes the arrays). */
+int vmc_problem_sources(const vmc_problem_desc *desc, int64_t *faces, int64_t *mothers, double *cdf,
+                        int64_t capacity, int64_t *nsources);
+
 /* Overrides for a variant of a resident problem; NULL keeps the base's. */
 typedef struct vmc_variant_desc {
   const double *mua, *mus, *g, *n;   /* [Ne]                                      */

@@ -617,6 +617,22 @@ int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsour
   return host_out_of_memory();
 }
 
+int vmc_problem_sources(const vmc_problem_desc *d, int64_t *faces, int64_t *mothers, double *cdf,
+                        int64_t capacity, int64_t *nsources) try {
+  if (!d || !nsources || capacity < 0) return VMC_ERR_INVALID_ARGUMENT;
+  vmc_problem m;
+  const int st = host_prepare(d, &m);
+  if (st != VMC_OK) return st;
+  const int64_t ns = (int64_t)m.lsFace.size(), k = std::min(ns, capacity);
+  if (faces) std::copy(m.lsFace.begin(), m.lsFace.begin() + k, faces);
+  if (mothers) std::copy(m.lsMother.begin(), m.lsMother.begin() + k, mothers);
+  if (cdf) std::copy(m.lsCDF.begin(), m.lsCDF.begin() + k, cdf);
+  *nsources = ns;
+  return VMC_OK;
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
+}
+
 int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) try {
   if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;

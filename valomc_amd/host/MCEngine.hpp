@@ -6,6 +6,10 @@
 // read the normalised ER/EI/EBR/EBI/DEBR/DEBI and loss.  The photon-level
 // methods of the reference (CreatePhoton, PropagatePhoton, WhichFace, ...)
 // run on the GPU inside MonteCarlo() and have no host counterpart here.
+// The host-side helpers keep their reference meaning: the light-source list
+// (BuildLightSource, filled by Init), the random stream `rng` with
+// InitRand/Unif*, the geometry queries of MC3D/MC2D (in MC3D.hpp, MC2D.hpp)
+// and the sharing assignment operator.
 // Two members are added: `precision` (VMC_FP64 by default, the reference's
 // arithmetic, or VMC_FP32) and `tally` (VMC_TALLY_FP64 or
 // VMC_TALLY_FIXED64), plus `stats` with the device counters of the last run.
@@ -15,19 +19,62 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "Array.hpp"
 #include "Errors.hpp"
+#include "mt_rng.hpp"
 #include "valomc.h"
 
 template <int DIM> class MCEngine {
 public:
+  // light-source indices: int in MC3D (MC3D.hpp:155-156), int_fast64_t in
+  // MC2D (MC2D.hpp:161-162)
+  typedef typename std::conditional<DIM == 3, int, int_fast64_t>::type SourceIndex;
+
   MCEngine()
       : f(0.0), omega(0.0), phase0(0.0), Nphoton(1), c0(2.99792458e11), weight0(0.001), chance(0.1),
         loss(0), seed(5489UL), threadcount(1), rank(0), nodecount(1), totalthreads(1),
         precision(VMC_FP64), tally(VMC_TALLY_FP64) {}
   virtual ~MCEngine() {}
+
+  // Assignment shares the problem (geometry, medium, boundary, light
+  // sources: Array copies are views) and gives the left-hand side its own
+  // zeroed outputs and a random stream reseeded from seed, as the
+  // reference's per-thread clones (MC3D.hpp:200-273).
+  MCEngine &operator=(const MCEngine &ref) {
+    if (this == &ref) return *this;
+    H = ref.H; HN = ref.HN; BH = ref.BH; r = ref.r;
+    mua = ref.mua; mus = ref.mus; g = ref.g; n = ref.n; k = ref.k; g2 = ref.g2;
+    BCType = ref.BCType; BCLNormal = ref.BCLNormal; BCIntensity = ref.BCIntensity;
+    BCLightDirectionType = ref.BCLightDirectionType; BCn = ref.BCn; GaussianSigma = ref.GaussianSigma;
+    f = ref.f; omega = ref.omega; phase0 = ref.phase0; Nphoton = ref.Nphoton; c0 = ref.c0;
+    ER.resize(ref.ER.N); EI.resize(ref.EI.N);
+    EBR.resize(ref.EBR.N); EBI.resize(ref.EBI.N);
+    DEBR.resize(ref.DEBR.N); DEBI.resize(ref.DEBI.N);   // resize() zero-fills
+    if (!BCIntensity.N) {
+      BCIntensity.resize(BCType.N);
+      for (int_fast64_t i = 0; i < BCIntensity.N; ++i) BCIntensity[i] = 1.0;
+    }
+    LightSources = ref.LightSources;
+    LightSourcesMother = ref.LightSourcesMother;
+    LightSourcesCDF = ref.LightSourcesCDF;
+    weight0 = ref.weight0; chance = ref.chance; loss = ref.loss;
+    threadcount = ref.threadcount; rank = ref.rank; nodecount = ref.nodecount; totalthreads = ref.totalthreads;
+    precision = ref.precision; tally = ref.tally;
+    seed = ref.seed;
+    InitRand();
+    return *this;
+  }
+
+  // random stream of host code (MC3D.hpp:276-303); the transport itself
+  // draws from per-photon streams on the GPU
+  void InitRand() { rng.Seed(seed); }
+  double UnifClosed() { return rng.drand_closed(); }      // [0, 1]
+  double UnifOpen() { return rng.drand_open(); }          // ]0, 1[
+  double UnifHalfDown() { return rng.drand_open_down(); } // ]0, 1]
+  double UnifHalfUp() { return rng.drand_open_up(); }     // [0, 1[
 
   // Size checks of ErrorChecks (MC3D.hpp:460-571, MC2D.hpp:390-513); the
   // content checks run in Init() through the C ABI, in the same order.
@@ -81,6 +128,29 @@ public:
     g2.resize(g.N);
     for (int_fast64_t i = 0; i < n.N; ++i) k[i] = omega / c0 * n[i];
     for (int_fast64_t i = 0; i < g.N; ++i) g2[i] = pow(g[i], 2);
+    BuildLightSource();
+  }
+
+  // Light sources of the boundary conditions (MC3D.hpp:941-1002): the
+  // emitting boundary faces in element-major order, their elements and the
+  // normalised CDF of area (2D: length) times BCIntensity.  Kept once built,
+  // as in the reference.
+  void BuildLightSource() {
+    if (LightSources.N != 0) return;
+    vmc_problem_desc d = desc();
+    int64_t ns = 0;
+    vmc_throw_status(vmc_problem_sources(&d, nullptr, nullptr, nullptr, 0, &ns));
+    std::vector<int64_t> face((size_t)ns), mother((size_t)ns);
+    std::vector<double> cdf((size_t)ns);
+    vmc_throw_status(vmc_problem_sources(&d, face.data(), mother.data(), cdf.data(), ns, &ns));
+    LightSources.resize(ns);
+    LightSourcesMother.resize(ns);
+    LightSourcesCDF.resize(ns);
+    for (int64_t i = 0; i < ns; ++i) {
+      LightSources[i] = (SourceIndex)face[i];
+      LightSourcesMother[i] = (SourceIndex)mother[i];
+      LightSourcesCDF[i] = cdf[i];
+    }
   }
 
   void BuildNeighbourhoods() {
@@ -123,9 +193,6 @@ public:
     if (progress) progress(100);
   }
 
-  double ElementVolume(int_fast64_t el);    // 3D: tet volume, 2D: triangle area
-  double ElementArea(int_fast64_t ib);      // 3D: triangle area, 2D: edge length
-
 public:
   Array<int_fast64_t> H, HN, BH;
   Array<double> r;
@@ -143,8 +210,12 @@ public:
   Array<double> ER, EI;
   Array<double> EBR, EBI;
   Array<double> DEBR, DEBI;
+  Array<SourceIndex> LightSources;
+  Array<SourceIndex> LightSourcesMother;
+  Array<double> LightSourcesCDF;
   double weight0, chance;
   long loss;
+  mt_rng rng;
   unsigned long seed;
   int threadcount;
   int rank, nodecount;
@@ -164,6 +235,7 @@ private:
     }
   };
 
+protected:
   vmc_problem_desc desc() {
     vmc_problem_desc d = {};
     d.dim = DIM;
@@ -184,33 +256,5 @@ private:
     return d;
   }
 };
-
-// geometry with the reference's expressions (MC3D.hpp:306-343, MC2D.hpp:303-318)
-template <> inline double MCEngine<3>::ElementVolume(int_fast64_t el) {
-  double ax = r(H(el, 0), 0), ay = r(H(el, 0), 1), az = r(H(el, 0), 2);
-  double bx = r(H(el, 1), 0), by = r(H(el, 1), 1), bz = r(H(el, 1), 2);
-  double cx = r(H(el, 2), 0), cy = r(H(el, 2), 1), cz = r(H(el, 2), 2);
-  double dx = r(H(el, 3), 0), dy = r(H(el, 3), 1), dz = r(H(el, 3), 2);
-  double v = (ax - dx) * ((by - dy) * (cz - dz) - (bz - dz) * (cy - dy)) -
-             (ay - dy) * ((bx - dx) * (cz - dz) - (bz - dz) * (cx - dx)) +
-             (az - dz) * ((bx - dx) * (cy - dy) - (by - dy) * (cx - dx));
-  return fabs(v) / 6.0;
-}
-template <> inline double MCEngine<3>::ElementArea(int_fast64_t ib) {
-  int_fast64_t i0 = BH(ib, 0), i1 = BH(ib, 1), i2 = BH(ib, 2);
-  double a = sqrt(pow(r(i1, 0) - r(i0, 0), 2) + pow(r(i1, 1) - r(i0, 1), 2) + pow(r(i1, 2) - r(i0, 2), 2));
-  double b = sqrt(pow(r(i2, 0) - r(i0, 0), 2) + pow(r(i2, 1) - r(i0, 1), 2) + pow(r(i2, 2) - r(i0, 2), 2));
-  double c = sqrt(pow(r(i1, 0) - r(i2, 0), 2) + pow(r(i1, 1) - r(i2, 1), 2) + pow(r(i1, 2) - r(i2, 2), 2));
-  return sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c)) / 4.0;
-}
-template <> inline double MCEngine<2>::ElementVolume(int_fast64_t el) {
-  double ax = r(H(el, 1), 0) - r(H(el, 0), 0), ay = r(H(el, 1), 1) - r(H(el, 0), 1);
-  double bx = r(H(el, 2), 0) - r(H(el, 0), 0), by = r(H(el, 2), 1) - r(H(el, 0), 1);
-  double c = ax * by - ay * bx;
-  return 0.5 * (c >= 0.0 ? c : -c);
-}
-template <> inline double MCEngine<2>::ElementArea(int_fast64_t ib) {
-  return sqrt(pow(r(BH(ib, 1), 0) - r(BH(ib, 0), 0), 2) + pow(r(BH(ib, 1), 1) - r(BH(ib, 0), 1), 2));
-}
 
 #endif
