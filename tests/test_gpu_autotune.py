@@ -36,7 +36,7 @@ def test_autotuned_settings_are_valid_and_keep_results(vmc, name, precision):
         assert t["trials"] >= 10
         blocks = (256, 128, 64) if prec == vmc.FP64 else (1024, 512, 256)
         assert t["block_size"] in blocks
-        assert t["refill_min"] in (6, 2, 16, 32)
+        assert t["refill_min"] in (6, 10, 16, 24, 32)
         assert t["chunk"] in (128, 64, 256, 512)
         assert t["replicas"] in (0, 8, 1)
         assert t["lds_slots"] in (0, -1)

@@ -225,7 +225,7 @@ def test_batch_autotune_keeps_results(vmc):
         t = vmc.batch_autotune(items, trial_fraction=0.5, **common)
         assert t.pop("trials") >= 10
         assert t["block_size"] in (1024, 512, 256) and t["chunk"] in (128, 64, 256, 512)
-        assert t["refill_min"] in (6, 2, 16, 32) and t["replicas"] in (0, 8, 1) and t["lds_slots"] in (0, -1)
+        assert t["refill_min"] in (6, 10, 16, 24, 32) and t["replicas"] in (0, 8, 1) and t["lds_slots"] in (0, -1)
         assert t["cache_epoch"] in (0, -1, 256, 4096)
         base, _ = vmc.batch_run(items, **common)
         tuned, _ = vmc.batch_run(items, **common, **t)
