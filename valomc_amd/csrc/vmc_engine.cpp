@@ -390,7 +390,7 @@ int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &
             upload_params(m, g.c.f64);
             g.tb.push_back(g.ne);
             g.bb.push_back(g.nb);
-            g.ne += sector_round(m.Ne);
+            g.ne += m.md->slots;
             g.nb += m.Nb;
           }
           if (g.ne > ((int64_t)1 << 30) / g.c.W || g.nb > ((int64_t)1 << 29)) return VMC_ERR_INVALID_ARGUMENT;
@@ -513,7 +513,7 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
   thread_local std::vector<unsigned long long> keep_loss;
   std::vector<double> &raw_e = use_scratch ? keep_e : own_e, &raw_b = use_scratch ? keep_b : own_b;
   std::vector<unsigned long long> &loss = use_scratch ? keep_loss : own_loss;
-  const int64_t ne = sector_round(m->Ne), nb = m->Nb;
+  const int64_t ne = m->md->slots, nb = m->Nb;
   try {
     hip_check(hipSetDevice(m->device), "hipSetDevice");
     upload_params(*m, c.f64);

@@ -133,7 +133,8 @@ struct MeshData {
   std::vector<double> r;
   std::vector<double> vol, area;    // element volumes (2D areas), boundary areas (2D lengths)
   std::vector<i4> nbcode;           // per element (caller's numbering), neighbours as device ids
-  std::vector<int32_t> perm, inv;   // device order: perm[device id] = element, inv[element] = device id
+  std::vector<int32_t> perm, inv;   // device order: perm[device slot] = element (-1: padding), inv[element] = slot
+  int64_t slots = 0;                // device slots: element records and tally words per replica, whole sectors
 };
 
 // Optical parameters and the device element records packed from them (with

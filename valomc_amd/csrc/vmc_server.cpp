@@ -460,7 +460,7 @@ int vmc_server_submit(vmc_server *s, vmc_problem *prob, const vmc_run_desc *run,
   r->run.tally = s->desc.tally;
   if (out) r->out = *out;
   r->fd = 2.0 * M_PI * run->f > 0.0;
-  r->ewords = sector_round(prob->Ne) * (r->fd ? 2 : 1);
+  r->ewords = prob->md->slots * (r->fd ? 2 : 1);
   if (r->ewords > s->cap_words || prob->Nb > s->cap_b) {
     g_last_error = "request larger than the server's tally arena";
     return VMC_ERR_OUT_OF_MEMORY;
