@@ -179,7 +179,8 @@ int vmc_problem_run(vmc_problem *prob, const vmc_run_desc *run, vmc_output *out,
      Otherwise VMC_ERR_INVALID_ARGUMENT.
    - A run whose tallies do not fit the scratch (one replica per XCD is
      reduced to a single one first) returns VMC_ERR_OUT_OF_MEMORY: up to
-     about 2M tetrahedra in CW or FD mode.
+     about 4M device element slots in FD mode and 8M in CW (a grid mesh
+     takes 4 slots per 3 tetrahedra: about 3M and 6M tetrahedra).
    - Every call on a GPU shares that GPU's scratch and clears what it uses
      at its start; overlapping calls on one GPU would race on it, so the
      library serialises the calls of a process with a per-GPU lock. */

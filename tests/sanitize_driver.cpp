@@ -2,7 +2,8 @@
 // ErrorChecks/Init pre-pass on up to 16 threads: neighbour matching, Morton
 // order, light sources, records; and the device-free entry points) and the
 // oracle, compiled with AddressSanitizer and UndefinedBehaviorSanitizer, on
-// a problem read from files.  Prints "key value" lines; a sanitizer report
+// a problem read from files; also the device element order the pre-pass
+// builds.  Prints "key value" lines; a sanitizer report
 // ends the program with an error.
 //   sanitize_driver <dir> <dim> <Ne> <Nb> <Nr> <nphoton>   (nphoton < 0: no oracle run)
 #include <cstdio>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "valomc.h"
+#include "vmc_host.h"
 #include "vmc_oracle.h"
 
 template <class T> static std::vector<T> load(const std::string &path, size_t n) {
@@ -74,6 +76,40 @@ int main(int argc, char **argv) {
   pd.H = badH.data();
   printf("bad_h %d\n", vmc_problem_check(&pd, nullptr, nullptr));
   pd.H = H.data();
+  {
+    // the device order the pre-pass builds: slots of whole sectors, every
+    // element in one slot and back, no neighbour code or light source on a
+    // padding slot; and, for a grid mesh whose element e lies in cell
+    // e % (Ne / k) (k = 6 tetrahedra or 2 triangles per cell), how many
+    // cells are split over two sectors
+    vmc_problem m;
+    const int st = vmc::host_prepare(&pd, &m);
+    const int64_t slots = m.md->slots;
+    int64_t used = 0, mismatch = 0, to_padding = 0, split = 0;
+    for (int32_t e : m.perm) used += e >= 0;
+    for (int64_t e = 0; e < Ne; ++e)
+      if (m.inv[e] < 0 || m.inv[e] >= slots || m.perm[m.inv[e]] != e) ++mismatch;
+    if (st == 0) {
+      for (int64_t e = 0; e < Ne; ++e)
+        for (int f = 0; f <= dim; ++f) {
+          const int c = m.nbcode[e][f];
+          if (c >= 0 && m.perm[c >> 3] < 0) ++to_padding;
+        }
+      for (const auto &src : m.src)
+        if (m.perm[src.mother] < 0) ++to_padding;
+    }
+    const int k = dim == 3 ? 6 : 2;
+    if (Ne % k == 0) {
+      const int64_t nc = Ne / k;
+      for (int64_t c = 0; c < nc; ++c) {
+        bool two = false;
+        for (int t = 1; t < k; ++t) two = two || m.inv[c + t * nc] / vmc::kSectorElems != m.inv[c] / vmc::kSectorElems;
+        split += two ? 1 : 0;
+      }
+    }
+    printf("order %d %lld %lld %lld %lld %lld\n", st, (long long)slots, (long long)used, (long long)mismatch,
+           (long long)to_padding, (long long)split);
+  }
 
   if (N < 0) return 0;   // host pre-pass only
   oracle_problem op = {};

@@ -8,7 +8,10 @@ entry points and the error paths) and the oracle are compiled with
 -fsanitize=address,undefined (host side only; the GPU kernels are the
 regular build's) and linked into tests/sanitize_driver.cpp, which runs them
 on golden cases and on the benchmark meshes, the 2M-tet frequency-domain box
-among them.  Any sanitizer report fails the run.
+among them.  Any sanitizer report fails the run.  The driver also reports
+the device element order the pre-pass builds, checked here: whole sectors,
+element and slot maps inverse to each other, nothing pointing at a padding
+slot, and on the grid meshes every cell inside one sector.
 """
 import os
 import subprocess
@@ -45,11 +48,15 @@ def driver(tmp_path_factory):
                                os.path.join(ROOT, "valomc_amd", "csrc", name + ".cpp"), "-o", o],
                               stderr=subprocess.DEVNULL)
         objs.append(o)
-    for src, extra in ((os.path.join(ROOT, "oracle", "vmc_oracle.cpp"), ["-fopenmp"]),
-                       (os.path.join(ROOT, "tests", "sanitize_driver.cpp"), [])):
-        o = str(d / (os.path.basename(src) + ".o"))
-        subprocess.check_call([CLANG, "-O1", "-g", "-std=c++17", *CPU_SAN, *extra, *inc, "-c", src, "-o", o])
-        objs.append(o)
+    o = str(d / "vmc_oracle.o")
+    subprocess.check_call([CLANG, "-O1", "-g", "-std=c++17", *CPU_SAN, "-fopenmp", *inc, "-c",
+                           os.path.join(ROOT, "oracle", "vmc_oracle.cpp"), "-o", o])
+    objs.append(o)
+    o = str(d / "sanitize_driver.o")   # includes the library's internal header: HIP host code
+    subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-O1", "-g", "-std=c++17", "-Wno-unused-command-line-argument",
+                           *HOST_SAN, *inc, "-x", "hip", "-c", os.path.join(ROOT, "tests", "sanitize_driver.cpp"),
+                           "-o", o], stderr=subprocess.DEVNULL)
+    objs.append(o)
     exe = str(d / "sanitize_driver")
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", *HOST_SAN, "-fopenmp", *objs, kernels, "-o", exe],
                           stderr=subprocess.DEVNULL)
@@ -80,8 +87,12 @@ def _run(driver, p, tmp, nphoton):
     return out
 
 
-def _check(out, nphoton, expect_sources=True):
+def _check(out, nphoton, expect_sources=True, grid=False):
     assert out["check"] == [["0"]]
+    st, slots, used, mismatch, to_padding, split = (int(x) for x in out["order"][0])
+    assert st == 0 and slots % 8 == 0 and mismatch == 0 and to_padding == 0, out["order"]
+    if grid:   # whole cells per sector: a 3D grid takes 4 slots per 3 tets, a 2D one fills its sectors
+        assert split == 0 and (4 * used == 3 * slots or (used + 7) // 8 * 8 == slots), out["order"]
     if expect_sources:
         assert int(out["nsources"][0][0]) > 0
         assert out["sources"][0] == ["0", out["nsources"][0][0]]
@@ -107,8 +118,8 @@ def test_benchmark_meshes_under_sanitizers(driver, tmp_path):
     box.BCType[:] = ord("a")
     top = np.all(np.abs(np.asarray(box.r)[np.asarray(box.BH)][:, :, 2] - 20.0) < 1e-9, axis=1)
     box.BCType[top] = ord("l")
-    _check(_run(driver, box, str(tmp_path), 300), 300)
+    _check(_run(driver, box, str(tmp_path), 300), 300, grid=True)
     fd = meshgen.fd_box_3d(nphoton=1, seed=7)
-    _check(_run(driver, fd, str(tmp_path), -1), -1)
+    _check(_run(driver, fd, str(tmp_path), -1), -1, grid=True)
     for q in meshgen.serving_pool(seed=2024, scale=0.05):
-        _check(_run(driver, q, str(tmp_path), 500), 500)
+        _check(_run(driver, q, str(tmp_path), 500), 500, grid=True)
