@@ -813,17 +813,24 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         bn = best ? num : bn;
         bc = best ? c : bc;
         bf = best ? f : bf;
-        code = best ? nb[f] : code;
-        mus_n = best ? nbo[f].x : mus_n;
-        n_n = best ? nbo[f].y : n_n;
       }
       // the record's second half (optical and neighbour words) is used only
       // past this branch; naming it here keeps its loads issued together
-      // with the planes' instead of after the face test
-      asm volatile("" ::"v"(opt.x), "v"(opt.y), "v"(opt.z), "v"(opt.w), "v"(nb.x), "v"(nb.y), "v"(nb.z));
-      if constexpr (D == 3) asm volatile("" ::"v"(nb.w));
+      // with the planes' instead of after the face test, and passing the
+      // test's result through keeps the wait for them after the test (the
+      // test needs the planes only, whose loads are issued first and return
+      // first), so the second half arrives while the test runs
+      asm volatile("" : "+v"(bf) : "v"(opt.x), "v"(opt.y), "v"(opt.z), "v"(opt.w), "v"(nb.x), "v"(nb.y), "v"(nb.z));
+      if constexpr (D == 3) asm volatile("" : "+v"(bf) : "v"(nb.w));
 #pragma unroll
-      for (int f = 0; f < NF; ++f) asm volatile("" ::"v"(nbo[f].x), "v"(nbo[f].y));
+      for (int f = 0; f < NF; ++f) asm volatile("" : "+v"(bf) : "v"(nbo[f].x), "v"(nbo[f].y));
+      // the chosen face's neighbour words
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        code = bf == f ? nb[f] : code;
+        mus_n = bf == f ? nbo[f].x : mus_n;
+        n_n = bf == f ? nbo[f].y : n_n;
+      }
       if (out_cur || bf < 0 || (uint32_t)p.nsteps > P.step_cap) {
         // no face ahead: the reference's WhichFace miss (MC3D.hpp:1464-1468)
         lose();
