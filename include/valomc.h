@@ -240,6 +240,15 @@ int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *
 /* The neighbour table used (reference HN convention), [Ne*(dim+1)]. */
 int vmc_problem_get_hn(const vmc_problem *prob, int64_t *HN);
 
+/* Device element slots of a problem: whole tally sectors of 8, padding
+   slots included (a sector holds whole cells of the element order, so a
+   3D grid mesh takes 4 slots per 3 tetrahedra).  The element records (128
+   bytes per slot in FP32, 256 in FP64) and every tally replica (8 bytes per
+   slot, twice that in the frequency domain) are sized by it.  No
+   counterpart in the reference: for callers that budget device memory,
+   such as a cache of resident meshes. */
+int vmc_problem_slots(const vmc_problem *prob, int64_t *slots);
+
 /* Replace the per-element optical parameters of a resident problem
    (any pointer may be NULL to keep the current values). */
 int vmc_problem_set_medium(vmc_problem *prob, const double *mua, const double *mus,

@@ -73,6 +73,16 @@ def test_box_fp64_matches_oracle_photon_for_photon(vmc, box):
     assert abs(led[0] + led[1] + led[2] - led[3] + led[4] - p.Nphoton) <= 1e-9 * p.Nphoton
 
 
+def test_box_device_slots(vmc, box):
+    """vmc_problem_slots: whole cells per tally sector, so the 40^3-cube box
+    takes 8 slots per cube of six tetrahedra (4 per 3); the serving cache
+    sizes resident meshes from it."""
+    from valomc_amd import serving
+    p, prob = box
+    assert prob.slots == 64000 * 8 == p.Ne * 4 // 3
+    assert serving.device_bytes(p) == serving.device_bytes(p, prob.slots)
+
+
 def test_box_fp32_within_one_percent_of_fp64(vmc, box):
     p, prob = box
     a = prob.run(2_000_000, seed=5, precision=vmc.FP64)

@@ -112,6 +112,7 @@ def load_library():
                                         C.POINTER(Stats), C.c_void_p, C.c_void_p]
         lib.vmc_problem_run_noalloc.argtypes = lib.vmc_problem_run.argtypes
         lib.vmc_problem_get_hn.argtypes = [C.c_void_p, C.c_void_p]
+        lib.vmc_problem_slots.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
         lib.vmc_problem_set_medium.argtypes = [C.c_void_p] + [C.c_void_p] * 4
         lib.vmc_problem_destroy.argtypes = [C.c_void_p]
         lib.vmc_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Stats)]
@@ -328,6 +329,16 @@ class Problem:
         if st:
             raise VmcError(st, self.lib)
         return out.reshape(nv, self.Ne).T.copy()
+
+    @property
+    def slots(self) -> int:
+        """vmc_problem_slots: device element slots (padding included) that
+        size the element records and the tally replicas."""
+        n = C.c_int64(0)
+        st = self.lib.vmc_problem_slots(self._h, C.byref(n))
+        if st:
+            raise VmcError(st, self.lib)
+        return int(n.value)
 
     def set_medium(self, mua=None, mus=None, g=None, n=None):
         """vmc_problem_set_medium: new optical parameters per element (None

@@ -771,6 +771,12 @@ int vmc_problem_get_hn(const vmc_problem *m, int64_t *HN) try {
   return host_out_of_memory();
 }
 
+int vmc_problem_slots(const vmc_problem *m, int64_t *slots) {
+  if (!m || !slots) return VMC_ERR_INVALID_ARGUMENT;
+  *slots = m->md->slots;
+  return VMC_OK;
+}
+
 int vmc_problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_stats *stats,
                     vmc_progress_fn progress, void *user) try {
   return problem_run(m, run, out, stats, progress, user, false);

@@ -13,7 +13,7 @@ from __future__ import annotations
 import hashlib
 import time
 from collections import OrderedDict
-from typing import Dict, List
+from typing import Optional, Dict, List
 
 import numpy as np
 
@@ -35,10 +35,16 @@ def content_key(p) -> str:
     return h.hexdigest()
 
 
-def device_bytes(p) -> int:
-    """Resident element records of a problem (FP32 and FP64 copies)."""
+def device_bytes(p, slots: Optional[int] = None) -> int:
+    """Resident element records of a problem (FP32 and FP64 copies), one per
+    device slot: `slots` when the problem exists (Problem.slots), else the
+    estimate for a grid mesh (3D: 4 slots per 3 tetrahedra; 2D: whole
+    sectors of two-triangle cells, no padding)."""
+    if slots is None:
+        slots = -(-int(p.Ne) * 4 // 3) if p.dim == 3 else int(p.Ne)
+        slots = -(-slots // 8) * 8
     per = 128 + 256 if p.dim == 3 else 128 + 192
-    return int(p.Ne) * per + int(p.Nb) * 16
+    return int(slots) * per + int(p.Nb) * 16
 
 
 class MeshCache:
@@ -66,6 +72,7 @@ class MeshCache:
                 self.used -= b
                 del self.entries[k]
         prob = Problem(p)
+        need = device_bytes(p, prob.slots)
         self.entries[key] = [prob, need, 1]
         self.used += need
         return prob
