@@ -13,7 +13,9 @@ medium and sources), with photon counts the CPU oracle finishes in seconds:
   the oracle run with the item's seed and stream tag;
 * the serving mix (2D and 3D jittered meshes, CW and 100 MHz): every
   request served in the mix bit-equal to its solve alone, and served
-  requests against the oracle.
+  requests against the oracle;
+* the box and the FD box against the reference's own MT19937 streams:
+  z-scores of spatially binned tallies over independent seeds.
 """
 import numpy as np
 import pytest
@@ -107,6 +109,79 @@ def test_fd_box_fp64_matches_oracle_and_loses_nothing(vmc):
     assert a.loss == 0 and b.loss == 0
     for k in ("ER", "EI", "EBR", "EBI"):
         assert rel_l2(getattr(b, k), getattr(a, k)) < 1e-2, (k, rel_l2(getattr(b, k), getattr(a, k)))
+
+
+def _spatial_bins(p, size):
+    """Element -> index of the cube of side `size` (mm) holding its centroid."""
+    r, H = np.asarray(p.r), np.asarray(p.H)
+    q = np.floor((r[H].mean(axis=1) - r.min(axis=0)) / size).astype(np.int64)
+    dims = q.max(axis=0) + 1
+    return (q[:, 0] * dims[1] + q[:, 1]) * dims[2] + q[:, 2], int(np.prod(dims))
+
+
+def _binned_z(g, r, pilot, bins, nbins, K):
+    """z of the difference of the means of K runs a side, per bin of summed
+    tallies, over the bins a separate pilot run finds above 1e-3 of its
+    largest (choosing bins by the tested runs' own spread would favour bins
+    whose samples happened to agree).  Returns z and the Bonferroni bound
+    at 1e-3 for the largest |z| under a t distribution with 2(K-1) degrees
+    of freedom (the standard errors come from K runs a side)."""
+    from scipy import stats
+    G = np.array([np.bincount(bins, x, nbins) for x in g])
+    Rr = np.array([np.bincount(bins, x, nbins) for x in r])
+    pil = np.abs(np.bincount(bins, pilot, nbins))
+    ok = pil > 1e-3 * pil.max()
+    se = np.sqrt(G.var(0, ddof=1) / K + Rr.var(0, ddof=1) / K)
+    z = (G.mean(0)[ok] - Rr.mean(0)[ok]) / se[ok]
+    nu = 2 * (K - 1)
+    return z, stats.t.ppf(1.0 - 1e-3 / (2 * max(ok.sum(), 1)), nu), nu
+
+
+def _check_z(tag, z, zmax, nu, nmin):
+    print(f"{tag}: {z.size} bins, mean z^2 {np.mean(z ** 2):.3f} (t: {nu / (nu - 2):.3f}), "
+          f"max |z| {np.abs(z).max():.2f} (bound {zmax:.2f})", flush=True)
+    assert z.size >= nmin, (tag, z.size)
+    assert 0.6 < np.mean(z ** 2) < 1.8, (tag, np.mean(z ** 2))
+    assert np.abs(z).max() < zmax, (tag, np.abs(z).max())
+
+
+def test_box_z_scores_against_reference_streams(vmc, box):
+    """The full 384k-tet box against the reference's own MT19937 streams
+    (the oracle's MT mode reproduces the reference bit for bit): K seeds a
+    side; element fluence summed over 2 mm cubes (48 tets each) and exitance
+    per boundary face must differ by standard-normal-looking z.  A bias of
+    0.5 % in the fluence gives |z| above 20 here."""
+    import copy
+    p, prob = box
+    K, n = 8, 200000
+    g = [prob.run(n, seed=2000 + k, precision=vmc.FP64) for k in range(K)]
+    pilot = prob.run(n, seed=999, precision=vmc.FP64)
+    r = []
+    for k in range(K):
+        q = copy.copy(p)
+        q.Nphoton, q.seed = n, 7000 + 32 * k   # thread t of an MT run uses seed + t: no two runs share a stream
+        r.append(pyoracle.run(q, 16, pyoracle.RNG_MT19937))
+    bins, nb = _spatial_bins(p, 2.0)
+    _check_z("box ER", *_binned_z([x.ER for x in g], [x.ER for x in r], pilot.ER, bins, nb, K), 2000)
+    _check_z("box EBR", *_binned_z([x.EBR for x in g], [x.EBR for x in r], pilot.EBR, np.arange(p.Nb), p.Nb, K), 2000)
+
+
+def test_fd_box_z_scores_against_reference_streams(vmc):
+    """The 2M-tet frequency-domain box: ER and EI summed over 3 mm cubes
+    against the reference's MT19937 streams, as for the box."""
+    p = meshgen.fd_box_3d(nphoton=1, seed=7)
+    K, n = 6, 20000
+    with vmc.Problem(p) as prob:
+        g = [prob.run(n, seed=3000 + k, f=p.f, precision=vmc.FP64) for k in range(K)]
+        pilot = prob.run(n, seed=999, f=p.f, precision=vmc.FP64)
+    r = []
+    for k in range(K):
+        p.Nphoton, p.seed = n, 9000 + 32 * k
+        r.append(pyoracle.run(p, 16, pyoracle.RNG_MT19937))
+    bins, nb = _spatial_bins(p, 3.0)
+    for key in ("ER", "EI"):
+        _check_z(f"fd box {key}", *_binned_z([getattr(x, key) for x in g], [getattr(x, key) for x in r],
+                                              getattr(pilot, key), bins, nb, K), 1000)
 
 
 def _instance_problem(base, source, medium):
