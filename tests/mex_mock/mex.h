@@ -2,7 +2,7 @@
    complex) that valomc_amd/host/vmc_mex.hpp uses, so the gateways compile
    and run without MATLAB.  mexErrMsgTxt throws instead of unwinding into
    MATLAB; the base workspace holds one logical, abort_photonMC, whose value
-   the driver can script. */
+   the test program (driver.cpp) can script. */
 #ifndef VMC_TEST_MEX_MOCK_H
 #define VMC_TEST_MEX_MOCK_H
 

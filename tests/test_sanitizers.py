@@ -8,7 +8,7 @@ entry points and the error paths) and the oracle are compiled with
 -fsanitize=address,undefined (host side only; the GPU kernels are the
 regular build's) and linked into tests/sanitize_driver.cpp, which runs them
 on golden cases and on the benchmark meshes, the 2M-tet frequency-domain box
-among them.  Any sanitizer report fails the run.  The driver also reports
+among them.  Any sanitizer report fails the run.  The test program also reports
 the device element order the pre-pass builds, checked here: whole sectors,
 element and slot maps inverse to each other, nothing pointing at a padding
 slot, and on the grid meshes every cell inside one sector.
@@ -113,7 +113,7 @@ def test_benchmark_meshes_under_sanitizers(driver, tmp_path):
     """The host preparation of the 384k-tet box, the 2,035,200-tet FD box
     (16 host threads) and the serving pool's 2D and 3D meshes."""
     box = meshgen.sphere_box_3d(nphoton=1, seed=1234)
-    # the pencil source needs its position, which this driver does not pass:
+    # the pencil source needs its position, which the test program does not pass:
     # use the top face's direct light instead
     box.BCType[:] = ord("a")
     top = np.all(np.abs(np.asarray(box.r)[np.asarray(box.BH)][:, :, 2] - 20.0) < 1e-9, axis=1)
