@@ -162,3 +162,12 @@ def philox(ctr, key):
     o = (C.c_uint32 * 4)()
     lib.oracle_philox4x32_10(c, k, o)
     return list(o)
+
+
+def xoshiro(state, n):
+    """n outputs of the photon stream's xoshiro128** step from `state`."""
+    lib = _load("liboracle.so")
+    s = (C.c_uint32 * 4)(*state)
+    o = (C.c_uint32 * n)()
+    lib.oracle_xoshiro128ss(s, C.c_int(n), o)
+    return list(o)

@@ -1054,4 +1054,10 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
   philox_block(ctr, key, out);
 }
 
+void oracle_xoshiro128ss(const uint32_t state[4], int n, uint32_t *out) {
+  PhiloxStream st(0);
+  for (int k = 0; k < 4; ++k) st.xs[k] = state[k];
+  for (int i = 0; i < n; ++i) out[i] = st.next();
+}
+
 } // extern "C"

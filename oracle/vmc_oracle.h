@@ -77,6 +77,10 @@ int oracle_run2d(const oracle_problem *p, int nthreads, int rng_mode, oracle_res
 /* Philox4x32-10 block function (Salmon et al., SC'11), for known-answer tests. */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* n outputs of the per-photon xoshiro128** step started from state[4] (the
+   walk's stream after seeding), for known-answer tests. */
+void oracle_xoshiro128ss(const uint32_t state[4], int n, uint32_t *out);
+
 /* Reference harness entry points (oracle/_ref/libref3d.so, libref2d.so). */
 int ref3d_run(const oracle_problem *p, int nthreads, oracle_result *out);
 int ref2d_run(const oracle_problem *p, int nthreads, oracle_result *out);

@@ -65,6 +65,17 @@ def test_philox_known_answers():
         assert pyoracle.philox(ctr, key) == expect
 
 
+def test_xoshiro128ss_known_answers():
+    # xoshiro128** from state (1, 2, 3, 4): the first ten outputs of the
+    # authors' reference implementation (prng.di.unimi.it,
+    # xoshiro128starstar.c), as published with the rand_xoshiro crate's
+    # reference test.  The GPU draws the same stream (photon-for-photon
+    # parity with the oracle's Philox mode).
+    expect = [11520, 0, 5927040, 70819200, 2031721883, 1637235492, 1287239034,
+              3734860849, 3729100597, 4258142804]
+    assert pyoracle.xoshiro([1, 2, 3, 4], 10) == expect
+
+
 def test_mt19937_known_answer():
     # mt_rng == MT19937 with init_genrand seeding (cpp/3d/mt_rng.hpp:119-125);
     # the C++ standard fixes the 10000th output for seed 5489.
