@@ -171,6 +171,29 @@ def test_brute_force_neighbours_small():
             assert hn[e, f] == cand[0]
 
 
+def test_size_limits_are_refused_before_any_array_is_read():
+    """Counts outside what the device layout addresses (2^28 elements,
+    int32 boundary indices) or that no mesh has are refused with
+    VMC_ERR_INVALID_ARGUMENT before the library reads a single array, so
+    the descriptor can point at one-element buffers."""
+    lib = vmc.load_library()
+    lib.vmc_problem_check.argtypes = [C.POINTER(vmc.ProblemDesc), C.c_void_p, C.POINTER(C.c_int64)]
+    one_i, one_f, one_c = np.zeros(1, np.int64), np.zeros(1), np.zeros(1, np.uint8)
+    base = dict(dim=3, Ne=6, Nb=12, Nr=8)
+    for bad in (dict(Ne=2 ** 28), dict(Nb=2 ** 31 - 1), dict(dim=4), dict(dim=1), dict(Ne=0), dict(Nb=-1),
+                dict(Nr=0)):
+        d = vmc.ProblemDesc()
+        for k, v in {**base, **bad}.items():
+            setattr(d, k, v)
+        d.H = d.BH = one_i.ctypes.data
+        d.r = d.mua = d.mus = d.g = d.n = d.BCn = one_f.ctypes.data
+        d.BCType = one_c.ctypes.data
+        assert lib.vmc_problem_check(C.byref(d), None, None) == vmc.STATUS_INVALID_ARGUMENT, bad
+    d = vmc.ProblemDesc()
+    d.dim, d.Ne, d.Nb, d.Nr = 3, 6, 12, 8   # counts fine, arrays missing
+    assert lib.vmc_problem_check(C.byref(d), None, None) == vmc.STATUS_INVALID_ARGUMENT
+
+
 def test_binding_refuses_arrays_of_the_wrong_length():
     """The library reads Ne, Nb, Nr (times the dimension) values from every
     pointer it is given, so the binding checks each array's length first

@@ -69,6 +69,31 @@ def test_fp64_matches_oracle_on_same_streams(vmc, name, tally):
     assert abs(res.loss - ref.loss) <= max(2, p.Nphoton // 1000), (res.loss, ref.loss)
 
 
+@pytest.mark.parametrize("nphot", [0, 1, 2, 65])
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_square_cosinic", "p2_iso_gauss_fd"])
+def test_tiny_photon_counts_match_oracle(vmc, name, nphot):
+    """Ragged minimal runs: no photon (every divided tally is 0/0 = NaN,
+    as in the reference), one, two, and one more than a wave.  FP64 output
+    equals the oracle's on the same streams, NaNs in the same words, and
+    loses the same photons; the FP32 path gives the same NaN pattern."""
+    p = _golden(name)
+    p.Nphoton = nphot
+    print(f"case {name} N={nphot}", flush=True)
+    ref = pyoracle.run(p, 1, pyoracle.RNG_PHILOX)
+    res = _run(vmc, p, precision=vmc.FP64)
+    res32 = _run(vmc, p, precision=vmc.FP32)
+    for k in OUT:
+        a, a32, b = getattr(res, k), getattr(res32, k), getattr(ref, k)
+        assert np.array_equal(np.isnan(a), np.isnan(b)), k
+        assert np.array_equal(np.isnan(a32), np.isnan(b)), k
+        fin = ~np.isnan(b)
+        if nphot == 0:
+            assert np.all(a[fin] == 0) and np.all(a32[fin] == 0), k
+        elif np.linalg.norm(b[fin]) > 0:
+            assert rel_l2(a[fin], b[fin]) < 1e-9, (k, rel_l2(a[fin], b[fin]))
+    assert res.loss == ref.loss
+
+
 @pytest.mark.parametrize("name", sorted(cases.CASES))
 def test_ledger_closes(vmc, name):
     p = _golden(name)

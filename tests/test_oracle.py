@@ -126,6 +126,22 @@ def test_philox_mode_statistically_matches_mt():
 
 
 @pytest.mark.skipif(not pyoracle.have_reference(), reason="reference engine not built here")
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_direct_abs_fd", "p2_square_cosinic", "p2_iso_gauss_fd"])
+def test_zero_photons_match_reference(name):
+    """Nphoton = 0: the reference still normalises by N, so every tally
+    it divides becomes 0/0 = NaN and the undivided ones stay 0; the oracle
+    reproduces that output, NaNs included, word for word."""
+    p = cases.from_arrays(_golden(name))
+    p.Nphoton = 0
+    a = pyoracle.run(p, 2)
+    b = pyoracle.run_reference(p, 2)
+    for k in OUT:
+        x, y = getattr(a, k), getattr(b, k)
+        assert np.array_equal(x, y, equal_nan=True), k
+    assert a.loss == b.loss == 0
+
+
+@pytest.mark.skipif(not pyoracle.have_reference(), reason="reference engine not built here")
 def test_live_reference_matches_oracle_other_seed():
     p = cases.p3_cosinic_hetero_n()
     p.seed, p.Nphoton = 123456789, 7001
