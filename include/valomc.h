@@ -266,6 +266,8 @@ typedef struct vmc_batch_item {
   int64_t loss;          /* out: MC3D::loss of this item                     */
   int32_t status;        /* out                                              */
   int32_t reserved;
+  double ledger[5];      /* out: this item's raw weight ledger, as
+                            vmc_stats::ledger of its run alone              */
 } vmc_batch_item;
 
 /* Many forward solves in one persistent launch per (dimension, CW/FD)
@@ -364,8 +366,9 @@ int vmc_server_submit(vmc_server *srv, vmc_problem *prob, const vmc_run_desc *ru
 /* done = 1 once the ticket's outputs are written */
 int vmc_server_poll(vmc_server *srv, uint64_t ticket, int32_t *done);
 /* Blocks until the ticket is done and releases it; stats (optional) gets
-   photons, loss, kernel_ms of the launches it took part in and total_ms
-   from submit to completion.  Returns the request's status. */
+   photons, loss, the request's own weight ledger, kernel_ms of the launches
+   it took part in and total_ms from submit to completion.  Returns the
+   request's status. */
 int vmc_server_wait(vmc_server *srv, uint64_t ticket, vmc_stats *stats);
 /* Launches so far, photons launched, summed kernel time, queue lengths. */
 int vmc_server_counters(vmc_server *srv, int64_t *launches, int64_t *photons, double *busy_ms,

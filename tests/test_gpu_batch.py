@@ -39,6 +39,15 @@ MIX = [("p3_pencil_sphere", dict(nphoton=30000, seed=3)),
        ("p3_iso_mirror", dict(nphoton=0, seed=1))]
 
 
+def _check_ledger(r, alone, n, tol):
+    """An item's own weight ledger (absorbed, exited, killed, gained, lost)
+    closes on its photon count and equals the ledger of its run alone: the
+    same photons, summed in another order."""
+    led = np.asarray(r.ledger)
+    assert abs(led[0] + led[1] + led[2] - led[3] + led[4] - n) <= tol * max(n, 1), led
+    assert np.allclose(led, alone.ledger, rtol=1e-9, atol=1e-9 * max(n, 1)), (led, alone.ledger)
+
+
 def _mix(vmc):
     probs = {}
     items = []
@@ -59,12 +68,16 @@ def test_batch_items_equal_single_runs(vmc, precision):
         print(f"batch of {len(items)} items, {precision}, fixed-point tallies", flush=True)
         res, stats = vmc.batch_run(items, precision=prec, tally=vmc.TALLY_FIXED64)
         assert stats.photons == sum(kw["nphoton"] for _, kw in items)
+        # the batch's ledger is the sum of its items'
+        assert np.allclose(stats.ledger, np.sum([r.ledger for r in res], axis=0), rtol=1e-12, atol=1e-9)
         for (prob, kw), r in zip(items, res):
             kw = dict(kw)
-            alone = prob.run(kw.pop("nphoton"), precision=prec, tally=vmc.TALLY_FIXED64, **kw)
+            n = kw.pop("nphoton")
+            alone = prob.run(n, precision=prec, tally=vmc.TALLY_FIXED64, **kw)
             for k in OUT:
                 assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
             assert r.loss == alone.loss
+            _check_ledger(r, alone, n, 1e-6 if precision == "fp32" else 1e-9)
     finally:
         for _, prob in probs.values():
             prob.close()
@@ -78,6 +91,7 @@ def test_batch_fp64_tallies_match_single_runs(vmc):
             kw = dict(kw)
             n = kw.pop("nphoton")
             alone = prob.run(n, precision=vmc.FP64, **kw)
+            _check_ledger(r, alone, n, 1e-9)
             for k in OUT:
                 a, b = getattr(r, k), getattr(alone, k)
                 if n == 0:

@@ -222,9 +222,14 @@ def test_batch_of_256_items_equals_items_alone(vmc, batch):
     assert stats.photons == total
     for (prob, kw), r in zip(items, res):
         kw = dict(kw)
-        alone = prob.run(kw.pop("nphoton"), precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+        n = kw.pop("nphoton")
+        alone = prob.run(n, precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
         _same_bits(r, alone)
         assert r.loss == alone.loss == 0
+        # each instance's own weight ledger closes and equals its solve alone
+        led = np.asarray(r.ledger)
+        assert abs(led[0] + led[1] + led[2] - led[3] + led[4] - n) <= 1e-6 * n, led
+        assert np.allclose(led, alone.ledger, rtol=1e-9, atol=1e-9 * n), (led, alone.ledger)
 
 
 @pytest.mark.parametrize("tally", ["fp64", "fixed"])

@@ -79,6 +79,11 @@ def test_server_results_equal_single_runs(vmc, arena, policy):
             for k in OUT:
                 assert np.array_equal(getattr(a, k), getattr(b, k)), (name, kw, k)
             assert a.loss == b.loss
+            # the request's own weight ledger, summed over its launches
+            n = kw["nphoton"]
+            led = np.asarray(a.ledger)
+            assert abs(led[0] + led[1] + led[2] - led[3] + led[4] - n) <= 1e-6 * n, (name, led)
+            assert np.allclose(led, b.ledger, rtol=1e-9, atol=1e-9 * n), (name, led, b.ledger)
     finally:
         for _, prob in probs.values():
             prob.close()

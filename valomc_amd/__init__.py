@@ -229,7 +229,7 @@ class VariantDesc(C.Structure):
 
 class BatchItem(C.Structure):
     _fields_ = [("prob", C.c_void_p), ("run", RunDesc), ("out", Output), ("loss", C.c_int64),
-                ("status", C.c_int32), ("reserved", C.c_int32)]
+                ("status", C.c_int32), ("reserved", C.c_int32), ("ledger", C.c_double * 5)]
 
 
 class Problem:
@@ -425,7 +425,7 @@ def _batch_array(items, common):
 def _batch_results(arr, outs, s):
     return [RunResult(status=int(arr[k].status), loss=int(arr[k].loss), steps=int(s.steps),
                       kernel_ms=float(s.kernel_ms), total_ms=float(s.total_ms),
-                      ledger=np.array(list(s.ledger)), photons=int(arr[k].run.Nphoton),
+                      ledger=np.array(list(arr[k].ledger)), photons=int(arr[k].run.Nphoton),
                       grid_blocks=int(s.grid_blocks), block_size=int(s.block_size),
                       rechecks=int(s.rechecks), **out) for k, out in enumerate(outs)]
 
@@ -433,8 +433,9 @@ def _batch_results(arr, outs, s):
 def batch_run(items, **common):
     """vmc_batch_run: ``items`` is a list of ``(Problem, run_kwargs)``; keyword
     arguments in ``common`` apply to every item (precision, tally, tuning).
-    Returns one RunResult per item (``steps``, ``kernel_ms`` and the ledger
-    are the batch's) and the batch RunResult-style stats."""
+    Returns one RunResult per item (``steps`` and ``kernel_ms`` are the
+    batch's; ``ledger`` and ``loss`` are the item's own) and the batch's
+    vmc_stats, whose ledger is the sum of the items'."""
     lib = load_library()
     arr, outs = _batch_array(items, common)
     s = Stats()
@@ -561,7 +562,7 @@ class Server:
         if st:
             raise VmcError(st, self.lib)
         return RunResult(status=st, loss=int(s.loss), steps=0, kernel_ms=float(s.kernel_ms),
-                         total_ms=float(s.total_ms), ledger=np.zeros(5), photons=n, grid_blocks=0,
+                         total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)), photons=n, grid_blocks=0,
                          block_size=0, **out)
 
     def counters(self):

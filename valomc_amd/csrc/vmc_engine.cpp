@@ -124,7 +124,7 @@ void arena_alloc(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t ni
   a.btally.ensure((size_t)nb * 4 * c.nrep);
   a.raw_e.ensure((size_t)ne * c.W);
   a.raw_b.ensure((size_t)nb * 4);
-  a.ledger.ensure(8);
+  a.ledger.ensure(ledger_words(nitems));
   a.counters.ensure(8);
   a.item_loss.ensure(nitems);
   a.items.ensure(nitems);
@@ -134,7 +134,8 @@ void arena_fill(Arena &a, const LaunchCfg &c) {
   const size_t ew = (size_t)a.ne * c.W * c.nrep, bw = (size_t)a.nb * 4 * c.nrep;
   hip_check(launch_fill_u64(a.etally.p, ew, 0ull, a.stream), "fill");
   hip_check(launch_fill_u64(a.btally.p, bw, 0ull, a.stream), "fill");
-  hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), 8, 0ull, a.stream), "fill");
+  hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), ledger_words(a.nitems), 0ull, a.stream),
+            "fill");
   hip_check(launch_fill_u64(a.counters.p, 8, 0ull, a.stream), "fill");
   hip_check(launch_fill_u64(a.item_loss.p, a.nitems, 0ull, a.stream), "fill");
 }
@@ -225,24 +226,35 @@ void arena_reduce(Arena &a, const LaunchCfg &c) {
 }
 
 void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss) {
+                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss,
+                    std::vector<double> &item_led) {
   const size_t ew = (size_t)a.ne * c.W, bw = (size_t)a.nb * 4;
+  const size_t ni = std::max<size_t>(1, a.nitems);
   raw_e.resize(ew);
   raw_b.resize(bw);
   loss.resize(a.nitems);
+  item_led.resize(5 * ni);
   hip_check(hipMemcpyAsync(raw_e.data(), a.raw_e.p, ew * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(raw_b.data(), a.raw_b.p, bw * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
-  hip_check(hipMemcpyAsync(led, a.ledger.p, 5 * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+  hip_check(hipMemcpyAsync(item_led.data(), a.ledger.p, item_led.size() * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipMemcpyAsync(cnt, a.counters.p, kCounters * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   if (a.nitems)
     hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, a.nitems * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
   hip_check(hipStreamSynchronize(a.stream), "sync");
+  sum_ledgers(item_led, ni, led);
 }
 
 void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss) {
+                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss,
+                   std::vector<double> &item_led) {
   arena_reduce(a, c);
-  arena_copy_out(a, c, raw_e, raw_b, led, cnt, loss);
+  arena_copy_out(a, c, raw_e, raw_b, led, cnt, loss, item_led);
+}
+
+void sum_ledgers(const std::vector<double> &item_led, size_t nitems, double led[5]) {
+  for (int q = 0; q < 5; ++q) led[q] = 0.0;
+  for (size_t k = 0; k < nitems; ++k)
+    for (int q = 0; q < 5; ++q) led[q] += item_led[5 * k + q];
 }
 
 // Item record of a problem run: its device arrays, stream key and phase.
@@ -409,12 +421,13 @@ int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &
 // adds the group's counters and ledger to *tot.
 void finish_group(vmc_batch_item *items, const BatchGroup &g, const std::vector<double> &raw_e,
                   const std::vector<double> &raw_b, const double led[5], const unsigned long long cnt[kCounters],
-                  const std::vector<unsigned long long> &loss, vmc_stats *tot) {
+                  const std::vector<unsigned long long> &loss, const std::vector<double> &item_led, vmc_stats *tot) {
   for (size_t k = 0; k < g.idx.size(); ++k) {
     vmc_batch_item &b = items[g.idx[k]];
     finish_outputs(*b.prob, b.run, b.run.Nphoton, g.c.W, raw_e.data() + g.tb[k] * g.c.W,
                    raw_b.data() + g.bb[k] * 4, &b.out);
     b.loss = (int64_t)loss[k];
+    for (int q = 0; q < 5; ++q) b.ledger[q] = item_led[5 * k + q];
     tot->photons += b.run.Nphoton;
   }
   tot->steps += (int64_t)cnt[1];
@@ -513,6 +526,9 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
   thread_local std::vector<unsigned long long> keep_loss;
   std::vector<double> &raw_e = use_scratch ? keep_e : own_e, &raw_b = use_scratch ? keep_b : own_b;
   std::vector<unsigned long long> &loss = use_scratch ? keep_loss : own_loss;
+  thread_local std::vector<double> keep_led;
+  std::vector<double> own_led;
+  std::vector<double> &item_led = use_scratch ? keep_led : own_led;
   const int64_t ne = m->md->slots, nb = m->Nb;
   try {
     hip_check(hipSetDevice(m->device), "hipSetDevice");
@@ -542,7 +558,7 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
         break;
       }
     }
-    arena_collect(a, c, raw_e, raw_b, led, cnt, loss);
+    arena_collect(a, c, raw_e, raw_b, led, cnt, loss, item_led);
   } catch (HipFail f) {
     return fail_hip(f);
   }
@@ -875,10 +891,11 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
       kernel_ms += launch_items(g_batch_arena, m0->num_cus, g.c, recs, &grid);
       std::vector<double> raw_e, raw_b;
       std::vector<unsigned long long> loss;
+      std::vector<double> item_led;
       double led[5];
       unsigned long long cnt[kCounters];
-      arena_collect(g_batch_arena, g.c, raw_e, raw_b, led, cnt, loss);
-      finish_group(items, g, raw_e, raw_b, led, cnt, loss, &tot);
+      arena_collect(g_batch_arena, g.c, raw_e, raw_b, led, cnt, loss, item_led);
+      finish_group(items, g, raw_e, raw_b, led, cnt, loss, item_led, &tot);
       tot.grid_blocks = grid;
       if (g.c.fixed && cnt[4]) range_bad = true;
     }
@@ -1000,11 +1017,12 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
     for (auto &pg : pl->groups) {
       std::vector<double> raw_e, raw_b;
       std::vector<unsigned long long> loss;
+      std::vector<double> item_led;
       double led[5];
       unsigned long long cnt[kCounters];
-      arena_copy_out(pg->arena, pg->g.c, raw_e, raw_b, led, cnt, loss);
+      arena_copy_out(pg->arena, pg->g.c, raw_e, raw_b, led, cnt, loss, item_led);
       check_dispenser(cnt[0], pg->w);
-      finish_group(items, pg->g, raw_e, raw_b, led, cnt, loss, &tot);
+      finish_group(items, pg->g, raw_e, raw_b, led, cnt, loss, item_led, &tot);
       tot.grid_blocks = pg->w.grid;
       if (pg->g.c.fixed && cnt[4]) range_bad = true;
     }

@@ -16,6 +16,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 namespace vmc {
@@ -285,13 +286,20 @@ void enqueue_walk(Arena &a, const LaunchCfg &c, const WalkLaunch &w);
 float launch_items(Arena &a, int num_cus, const LaunchCfg &c, std::vector<ItemRec> &items, int *grid_out);
 
 // Enqueues the replica sums into raw element words [ne*W] and boundary
-// words [nb*4]; arena_copy_out copies them, the ledger, the counters and the
-// per-item loss counts to the host and waits.
+// words [nb*4]; arena_copy_out copies them, the item ledgers (and their sum
+// in led), the counters and the per-item loss counts to the host and waits.
 void arena_reduce(Arena &a, const LaunchCfg &c);
 void arena_copy_out(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss);
+                    double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss,
+                    std::vector<double> &item_led);
 void arena_collect(Arena &a, const LaunchCfg &c, std::vector<double> &raw_e, std::vector<double> &raw_b,
-                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss);   // reduce + copy out
+                   double led[5], unsigned long long cnt[kCounters], std::vector<unsigned long long> &loss,
+                   std::vector<double> &item_led);   // reduce + copy out
+// Ledger words of a launch of `nitems` items: five per item (absorbed,
+// exited, killed, gained, lost; one item: the launch's own), at least 8.
+inline size_t ledger_words(size_t nitems) { return std::max<size_t>(8, 5 * nitems); }
+// led = the sum of the first nitems item ledgers
+void sum_ledgers(const std::vector<double> &item_led, size_t nitems, double led[5]);
 ItemRec make_item(const vmc_problem &m, const vmc_run_desc *run, bool f64, uint64_t count, uint64_t id0,
                   int64_t tbase, int64_t bbase);
 void finish_outputs(const vmc_problem &m, const vmc_run_desc &run, int64_t N, int W, const double *raw_e,

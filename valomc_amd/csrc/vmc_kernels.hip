@@ -536,10 +536,16 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   uint32_t nrecheck = 0;        // FP32 exit-face choices decided in FP64
   bool alive = false;
   // absorbed weight accumulates per lane; the rare end-of-photon sums go to
-  // per-wave LDS slots [exited, killed, gained, lost, loss count]
+  // per-wave LDS slots [exited, killed, gained, lost, loss count].  A batch
+  // keeps one ledger per item (P.ledger[item * 5 + q]): the lane's absorbed
+  // weight belongs to its photon's item and leaves when the lane takes a
+  // photon of another item; the wave's slots sum for `led_item`, the item
+  // of its current photon pool, and an event of another item's photon goes
+  // straight to that item's ledger
   double l_abs = 0;
   __shared__ double s_led[MaxBlock<R>::value / 64][5];
   const int wid = threadIdx.x >> 6;
+  int led_item = 0;   // wave-uniform
   if (lane == 0) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) s_led[wid][q] = 0.0;
@@ -662,6 +668,28 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   int item = 0, tbase = BATCH ? 0 : P.items[0].tbase;
   R kf = BATCH ? R(0) : R(P.items[0].kfac);
 
+  // ledger event q (exited, killed, gained, lost) of the lane's photon
+  auto led_add = [&](int q, double v) {
+    if constexpr (BATCH) {
+      if (item != led_item) {
+        unsafeAtomicAdd(&P.ledger[(size_t)item * 5 + 1 + q], v);
+        return;
+      }
+    }
+    atomicAdd(&s_led[wid][q], v);
+  };
+  // lane 0: moves the wave's slots to entries 1..4 of item `it`'s ledger
+  // (batch: and clears them for the next item)
+  auto flush_wave_ledger = [&](int it) {
+    double *const L = P.ledger + (BATCH ? (size_t)it * 5 : 0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double v = s_led[wid][q];
+      if constexpr (BATCH) s_led[wid][q] = 0.0;
+      if (v != 0.0) unsafeAtomicAdd(&L[1 + q], v);
+    }
+  };
+
   // wave-private pool of virtual photon ids [pool_next, pool_end), all of
   // item pool_item; a chunk never straddles two items
   uint64_t pool_next = 0, pool_end = 0;
@@ -709,6 +737,12 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           pool_next = fresh + (ndead - avail);
           pool_end = fresh + (uint64_t)P.chunk;
           pool_item = fresh_item;
+          if constexpr (BATCH) {
+            if (fresh < P.vtotal && fresh_item != led_item) {
+              if (lane == 0) flush_wave_ledger(led_item);
+              led_item = fresh_item;
+            }
+          }
         } else {
           pool_next += ndead;
         }
@@ -721,6 +755,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
             const uint64_t sid = it.id0 + k;
             rs.seed((uint32_t)sid, (uint32_t)(sid >> 32), it);
             if constexpr (BATCH) {
+              if (pit != item) {   // the absorbed weight so far is the old item's
+                if (l_abs != 0.0) unsafeAtomicAdd(&P.ledger[item * 5], l_abs);
+                l_abs = 0.0;
+              }
               item = pit;
               E = (ElemPtr<NF, R>)(it.elems);
               tbase = it.tbase;
@@ -778,7 +816,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       };
       auto lose = [&]() {
         end_visit();
-        atomicAdd(&s_led[wid][3], (double)p.w);
+        led_add(3, (double)p.w);
         atomicAdd(&s_led[wid][4], 1.0);
         atomicAdd(&P.item_loss[item], 1ull);
         alive = false;
@@ -985,7 +1023,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           }
         }
         end_visit();
-        atomicAdd(&s_led[wid][0], (double)p.w);
+        led_add(0, (double)p.w);
         alive = false;
         return;
       }
@@ -1007,12 +1045,12 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         // Russian roulette at the crossing (MC3D.hpp:1597-1604)
         if (Unif<R>::closed(rs.next(P)) > chance) {
           end_visit();
-          atomicAdd(&s_led[wid][1], (double)p.w);
+          led_add(1, (double)p.w);
           alive = false;
           return;
         }
         const R wn = p.w / chance;
-        atomicAdd(&s_led[wid][2], (double)wn - (double)p.w);
+        led_add(2, (double)wn - (double)p.w);
         p.w = wn;
       }
       if (n_c != n_n) {
@@ -1105,9 +1143,13 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   flush_sector(sect >= 0 && cslot < 0);
 
   // ---- flush the counters: one set of global atomics per wave ----
+  if constexpr (BATCH) {
+    if (l_abs != 0.0) unsafeAtomicAdd(&P.ledger[item * 5], l_abs);
+    l_abs = 0.0;
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    l_abs += __shfl_xor(l_abs, off);
+    if constexpr (!BATCH) l_abs += __shfl_xor(l_abs, off);
     nrecheck += __shfl_xor(nrecheck, off);
   }
   __syncthreads();
@@ -1119,11 +1161,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
     if (sc >= 0 && bits) replica_add(etally, (int64_t)sc * WS + i % WS, bits);
   }
   if (lane == 0) {
-    unsafeAtomicAdd(&P.ledger[0], l_abs);
-    unsafeAtomicAdd(&P.ledger[1], s_led[wid][0]);
-    unsafeAtomicAdd(&P.ledger[2], s_led[wid][1]);
-    unsafeAtomicAdd(&P.ledger[3], s_led[wid][2]);
-    unsafeAtomicAdd(&P.ledger[4], s_led[wid][3]);
+    if constexpr (!BATCH) unsafeAtomicAdd(&P.ledger[0], l_abs);
+    flush_wave_ledger(led_item);
     atomicAdd(&P.counters[1], (unsigned long long)steps);
     atomicAdd(&P.counters[2], (unsigned long long)s_led[wid][4]);
     if (nrecheck) atomicAdd(&P.counters[3], (unsigned long long)nrecheck);

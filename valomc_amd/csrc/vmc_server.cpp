@@ -43,6 +43,7 @@ struct Request {
   int64_t eoff = -1, ewords = 0; // element words in each arena replica
   int64_t boff = -1;             // first boundary slot
   int64_t loss = 0;
+  double ledger[5] = {0, 0, 0, 0, 0};   // absorbed, exited, killed, gained, lost over its launches
   double kernel_ms = 0;
   int status = VMC_OK;
   bool finished = false;
@@ -324,12 +325,18 @@ void server_loop(vmc_server *s) {
         a.nitems = items.size();
         a.items.ensure(items.size());
         a.item_loss.ensure(items.size());
+        a.ledger.ensure(ledger_words(items.size()));
         hip_check(launch_fill_u64(a.item_loss.p, items.size(), 0ull, a.stream), "fill");
+        hip_check(launch_fill_u64(reinterpret_cast<unsigned long long *>(a.ledger.p), ledger_words(items.size()), 0ull,
+                                  a.stream), "fill");
         int grid = 0;
         const float ms = launch_items(a, s->num_cus, c, items, &grid);
         std::vector<unsigned long long> loss(items.size());
+        std::vector<double> item_led(5 * items.size());
         unsigned long long range_flag = 0;
         hip_check(hipMemcpyAsync(loss.data(), a.item_loss.p, loss.size() * 8, hipMemcpyDeviceToHost, a.stream), "d2h");
+        hip_check(hipMemcpyAsync(item_led.data(), a.ledger.p, item_led.size() * 8, hipMemcpyDeviceToHost, a.stream),
+                  "d2h");
         hip_check(hipMemcpyAsync(&range_flag, a.counters.p + 4, 8, hipMemcpyDeviceToHost, a.stream), "d2h");
         hip_check(hipStreamSynchronize(a.stream), "sync");
         if (c.fixed && range_flag) {
@@ -355,6 +362,7 @@ void server_loop(vmc_server *s) {
           Request *r = group[k];
           r->launched += slice[k];
           r->loss += (int64_t)loss[k];
+          for (int q = 0; q < 5; ++q) r->ledger[q] += item_led[5 * k + q];
           r->kernel_ms += ms;
           s->photons += slice[k];
           if (r->launched >= r->run.Nphoton) done_now.push_back(r);
@@ -499,6 +507,7 @@ int vmc_server_wait(vmc_server *s, uint64_t ticket, vmc_stats *stats) try {
     std::memset(stats, 0, sizeof(*stats));
     stats->photons = r->run.Nphoton;
     stats->loss = r->loss;
+    for (int q = 0; q < 5; ++q) stats->ledger[q] = r->ledger[q];
     stats->kernel_ms = r->kernel_ms;
     stats->total_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
   }

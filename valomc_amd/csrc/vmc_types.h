@@ -139,7 +139,8 @@ struct WalkParams {
   uint64_t vtotal;              // virtual photon ids [0, vtotal) of this launch
   void *etally;
   void *btally;
-  double *ledger;               // [5] absorbed, exited, killed, gained, lost
+  double *ledger;               // [5] absorbed, exited, killed, gained, lost;
+                                // a batch launch: [nitems][5], one per item
   unsigned long long *counters; // [8] photon dispenser, steps, loss, rechecks, fixed-point range flag
   unsigned long long *item_loss; // [nitems] photons lost per item (MC3D::loss)
   int64_t ne, nb;               // arena element slots (whole sectors) and boundary slots
