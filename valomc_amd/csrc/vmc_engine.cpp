@@ -38,8 +38,10 @@ int run_model(const vmc_run_desc *run, RunModel *m) {
 
 // Validates a run's precision, tally, model and tuning fields and resolves
 // the launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of
-// LDS.
-int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c) {
+// LDS; `batch`: the launch may walk several items (their kernels keep four
+// ledger slots per wave in static LDS).
+int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c,
+                bool batch) {
   (void)num_cus;
   if (run->precision != VMC_FP64 && run->precision != VMC_FP32) return VMC_ERR_INVALID_ARGUMENT;
   if (run->tally != VMC_TALLY_FP64 && run->tally != VMC_TALLY_FIXED64) return VMC_ERR_INVALID_ARGUMENT;
@@ -73,7 +75,10 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
     const int waves_per_cu = c->f64 ? 8 : 16;
     const int bpc_target = std::max(1, waves_per_cu / (c->block / 64));
     const size_t per_block = (size_t)lds_per_cu / bpc_target;
-    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->block).total + 1024;   // + ledger, alignment
+    // + the kernel's static LDS (16 waves' ledgers: 40 bytes a wave for one
+    // item, 44 a slot + 8 for a batch) and alignment
+    const size_t wave_ledger = batch ? ((size_t)kBatchLedgerSlots * 44 + 8 + 7) / 8 * 8 : 40;
+    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->block).total + 16 * wave_ledger + 384;
     const size_t entry = (size_t)kSectorElems * c->W * 8 + 4;
     c->slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
   }
@@ -393,7 +398,7 @@ int plan_batch(const vmc_batch_item *items, int32_t n, std::vector<BatchGroup> &
           BatchGroup g;
           for (int32_t i = 0; i < n; ++i)
             if (in_class(i) && model[i].weight0 == pr.first && model[i].chance == pr.second) g.idx.push_back(i);
-          const int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &g.c);
+          const int st = resolve_cfg(dim, m0->num_cus, m0->lds_per_cu, fdi == 1, &items[0].run, &g.c, g.idx.size() > 1);
           if (st != VMC_OK) return st;
           g.c.weight0 = pr.first;
           g.c.chance = pr.second;
@@ -504,7 +509,7 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
   const auto t_start = std::chrono::steady_clock::now();
   const bool fd = 2.0 * M_PI * run->f > 0.0;
   LaunchCfg c;
-  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c);
+  const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, fd, run, &c, false);
   if (st != VMC_OK) return st;
   const int64_t N = run->Nphoton;
   if (c.fixed && N >= kFixedMaxPhotons) return VMC_ERR_INVALID_ARGUMENT;
@@ -812,7 +817,7 @@ int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *
   if (!m || !run || !tuned || run->Nphoton < 0 || trial_photons < 0) return VMC_ERR_INVALID_ARGUMENT;
   {
     LaunchCfg c;
-    const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, 2.0 * M_PI * run->f > 0.0, run, &c);
+    const int st = resolve_cfg(m->dim, m->num_cus, m->lds_per_cu, 2.0 * M_PI * run->f > 0.0, run, &c, false);
     if (st != VMC_OK) return st;
   }
   const int64_t n = trial_photons > 0 ? trial_photons

@@ -261,7 +261,8 @@ struct LaunchCfg {
   double weight0 = kWeight0, chance = kChance;   // roulette constants of the launch
 };
 
-int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c);
+int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c,
+                bool batch);
 void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb);
 // The arena's buffers for ne element and nb boundary slots and nitems
 // items (no device work; nothing is allocated under stream capture).

@@ -487,6 +487,41 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const I
 // The walk
 // ---------------------------------------------------------------------------
 
+// Per-wave LDS ledger of the walk, words in the ledger's order (0 absorbed,
+// 1 exited, 2 killed, 3 gained, 4 lost) plus the wave's loss count.  One
+// problem: a single slot whose absorbed word stays in the lanes (summed
+// across the wave at the end).  A batch: kSlots slots direct-mapped by item,
+// each with the item it sums for (-1: none).
+template <bool BATCH> struct WaveLedger;
+template <> struct WaveLedger<false> {
+  static constexpr int slots = 1, first = 1;
+  double v[5];   // exited, killed, gained, lost, loss count
+  __device__ double &word(int, int q) { return v[q - 1]; }
+  __device__ double &nloss() { return v[4]; }
+  __device__ int item_of(int) const { return 0; }
+  __device__ void set_item(int, int) {}
+  __device__ void clear() {
+    for (int q = 0; q < 5; ++q) v[q] = 0.0;
+  }
+};
+template <> struct WaveLedger<true> {
+  static constexpr int slots = kBatchLedgerSlots, first = 0;
+  double v[slots][5];
+  double nl;
+  int it[slots];
+  __device__ double &word(int sl, int q) { return v[sl][q]; }
+  __device__ double &nloss() { return nl; }
+  __device__ int item_of(int sl) const { return it[sl]; }
+  __device__ void set_item(int sl, int item) { it[sl] = item; }
+  __device__ void clear() {
+    for (int sl = 0; sl < slots; ++sl) {
+      for (int q = 0; q < 5; ++q) v[sl][q] = 0.0;
+      it[sl] = -1;
+    }
+    nl = 0.0;
+  }
+};
+
 // FP32 kernels fit 16 waves per CU in 128 VGPRs, so they take workgroups of
 // up to 1024 lanes (one large LDS tally cache per CU); FP64 kernels need more
 // registers per lane and stay at 256.
@@ -536,20 +571,21 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   uint32_t nrecheck = 0;        // FP32 exit-face choices decided in FP64
   bool alive = false;
   // absorbed weight accumulates per lane; the rare end-of-photon sums go to
-  // per-wave LDS slots [exited, killed, gained, lost, loss count].  A batch
-  // keeps one ledger per item (P.ledger[item * 5 + q]): the lane's absorbed
-  // weight belongs to its photon's item and leaves when the lane takes a
-  // photon of another item; the wave's slots sum for `led_item`, the item
-  // of its current photon pool, and an event of another item's photon goes
-  // straight to that item's ledger
+  // per-wave LDS slots [absorbed, exited, killed, gained, lost] (the ledger's
+  // order) and a per-wave loss count.  A batch keeps one ledger per item
+  // (P.ledger[item * 5 + q]) and each wave kLed slots, direct-mapped by
+  // item: a wave's pool takes the slot of its item when it moves to it,
+  // handing the sums of the item it held there to that item's ledger, so
+  // the photons still alive from the wave's last few pools (other items)
+  // keep adding in LDS; an event whose item lost its slot goes straight to
+  // the item's ledger.  A lane's absorbed weight leaves for its photon's item
+  // when the lane takes a photon of another item, and at the end.
   double l_abs = 0;
-  __shared__ double s_led[MaxBlock<R>::value / 64][5];
+  constexpr int kLed = WaveLedger<BATCH>::slots;
+  __shared__ WaveLedger<BATCH> s_wl[MaxBlock<R>::value / 64];
   const int wid = threadIdx.x >> 6;
-  int led_item = 0;   // wave-uniform
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) s_led[wid][q] = 0.0;
-  }
+  WaveLedger<BATCH> &wl = s_wl[wid];
+  if (lane == 0) wl.clear();
   constexpr int WS = kSectorElems * W;   // tally words per sector
   extern __shared__ __align__(16) unsigned char s_dyn[];
   const int S = P.lds_slots > 0 ? P.lds_slots : 0;
@@ -668,25 +704,36 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   int item = 0, tbase = BATCH ? 0 : P.items[0].tbase;
   R kf = BATCH ? R(0) : R(P.items[0].kfac);
 
-  // ledger event q (exited, killed, gained, lost) of the lane's photon
+  // lanes whose photon's item holds its ledger slot (wave-uniform mask,
+  // updated where slots and photons change hands: at refills)
+  uint64_t own_mask = 0;
+  // ledger word q (0 absorbed, 1 exited, 2 killed, 3 gained, 4 lost) of
+  // the lane's photon's item
+  // (address-space-typed pointers: the LDS and global adds stay ds_ and
+  // global_ atomics instead of merging into one flat atomic)
   auto led_add = [&](int q, double v) {
     if constexpr (BATCH) {
-      if (item != led_item) {
-        unsafeAtomicAdd(&P.ledger[(size_t)item * 5 + 1 + q], v);
+      if (!((own_mask >> lane) & 1ull)) {
+        auto *g = (__attribute__((address_space(1))) double *)(P.ledger + (size_t)item * 5 + q);
+        __hip_atomic_fetch_add(g, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
       }
+      auto *l = (__attribute__((address_space(3))) double *)&wl.word(item & (kLed - 1), q);
+      __hip_atomic_fetch_add(l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+      atomicAdd(&wl.word(0, q), v);
     }
-    atomicAdd(&s_led[wid][q], v);
   };
-  // lane 0: moves the wave's slots to entries 1..4 of item `it`'s ledger
-  // (batch: and clears them for the next item)
-  auto flush_wave_ledger = [&](int it) {
-    double *const L = P.ledger + (BATCH ? (size_t)it * 5 : 0);
+  // lane 0: moves slot sl's sums to its item's ledger and clears them
+  auto flush_led_slot = [&](int sl) {
+    const int it = wl.item_of(sl);
+    if (it < 0) return;
+    double *const L = P.ledger + (size_t)it * 5;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double v = s_led[wid][q];
-      if constexpr (BATCH) s_led[wid][q] = 0.0;
-      if (v != 0.0) unsafeAtomicAdd(&L[1 + q], v);
+    for (int q = WaveLedger<BATCH>::first; q < 5; ++q) {
+      const double v = wl.word(sl, q);
+      wl.word(sl, q) = 0.0;
+      if (v != 0.0) unsafeAtomicAdd(&L[q], v);
     }
   };
 
@@ -738,9 +785,16 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           pool_end = fresh + (uint64_t)P.chunk;
           pool_item = fresh_item;
           if constexpr (BATCH) {
-            if (fresh < P.vtotal && fresh_item != led_item) {
-              if (lane == 0) flush_wave_ledger(led_item);
-              led_item = fresh_item;
+            // the pool's item takes its ledger slot; the photons of the item
+            // that held it lose theirs
+            const int sl = fresh_item & (kLed - 1);
+            const int held = fresh < P.vtotal ? wl.item_of(sl) : fresh_item;
+            if (held != fresh_item) {
+              if (lane == 0) {
+                flush_led_slot(sl);
+                wl.set_item(sl, fresh_item);
+              }
+              own_mask &= ~__ballot(item == held);
             }
           }
         } else {
@@ -748,6 +802,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         }
         const bool invalid = !alive && pid >= P.vtotal;
         if (__ballot(invalid)) exhausted = true;
+        const bool was_alive = alive;
         if (!alive && !invalid) {
           const ItemRec &it = P.items[pit];
           const uint64_t k = pid - it.vstart;
@@ -756,7 +811,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
             rs.seed((uint32_t)sid, (uint32_t)(sid >> 32), it);
             if constexpr (BATCH) {
               if (pit != item) {   // the absorbed weight so far is the old item's
-                if (l_abs != 0.0) unsafeAtomicAdd(&P.ledger[item * 5], l_abs);
+                if (l_abs != 0.0) led_add(0, l_abs);
                 l_abs = 0.0;
               }
               item = pit;
@@ -768,6 +823,12 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
             p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
             alive = true;
           }
+        }
+        if constexpr (BATCH) {
+          // a new photon adds in LDS when its item holds the slot
+          const bool born = alive && !was_alive;
+          const bool mine = born && wl.item_of(item & (kLed - 1)) == item;
+          own_mask = (own_mask & ~__ballot(born)) | __ballot(mine);
         }
       }
     }
@@ -816,8 +877,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       };
       auto lose = [&]() {
         end_visit();
-        led_add(3, (double)p.w);
-        atomicAdd(&s_led[wid][4], 1.0);
+        led_add(4, (double)p.w);
+        atomicAdd(&wl.nloss(), 1.0);
         atomicAdd(&P.item_loss[item], 1ull);
         alive = false;
       };
@@ -1023,7 +1084,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           }
         }
         end_visit();
-        led_add(0, (double)p.w);
+        led_add(1, (double)p.w);
         alive = false;
         return;
       }
@@ -1045,12 +1106,12 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         // Russian roulette at the crossing (MC3D.hpp:1597-1604)
         if (Unif<R>::closed(rs.next(P)) > chance) {
           end_visit();
-          led_add(1, (double)p.w);
+          led_add(2, (double)p.w);
           alive = false;
           return;
         }
         const R wn = p.w / chance;
-        led_add(2, (double)wn - (double)p.w);
+        led_add(3, (double)wn - (double)p.w);
         p.w = wn;
       }
       if (n_c != n_n) {
@@ -1144,7 +1205,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
 
   // ---- flush the counters: one set of global atomics per wave ----
   if constexpr (BATCH) {
-    if (l_abs != 0.0) unsafeAtomicAdd(&P.ledger[item * 5], l_abs);
+    if (l_abs != 0.0) led_add(0, l_abs);
     l_abs = 0.0;
   }
 #pragma unroll
@@ -1162,9 +1223,10 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   }
   if (lane == 0) {
     if constexpr (!BATCH) unsafeAtomicAdd(&P.ledger[0], l_abs);
-    flush_wave_ledger(led_item);
+#pragma unroll
+    for (int sl = 0; sl < kLed; ++sl) flush_led_slot(sl);
     atomicAdd(&P.counters[1], (unsigned long long)steps);
-    atomicAdd(&P.counters[2], (unsigned long long)s_led[wid][4]);
+    atomicAdd(&P.counters[2], (unsigned long long)wl.nloss());
     if (nrecheck) atomicAdd(&P.counters[3], (unsigned long long)nrecheck);
   }
   if constexpr (FIXED) {

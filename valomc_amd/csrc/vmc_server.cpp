@@ -295,7 +295,7 @@ void server_loop(vmc_server *s) {
                                      (size_t)s->cap_b * 4, 0ull, a.stream), "fill");
       }
       if (!group.empty()) {
-        const int st = resolve_cfg(gdim, s->num_cus, s->lds_per_cu, gfd, &tmpl, &c);
+        const int st = resolve_cfg(gdim, s->num_cus, s->lds_per_cu, gfd, &tmpl, &c, true);
         if (st != VMC_OK) throw HipFail{hipErrorInvalidValue, "server launch configuration"};
         c.weight0 = group[0]->model.weight0;
         c.chance = group[0]->model.chance;

@@ -93,6 +93,13 @@ struct alignas(16) SrcRec {
 // tally words, one 64-byte line of the replica in continuous-wave mode.
 constexpr int kSectorElems = 8;
 
+// Ledger slots per wave of a batch walk (a power of 2): the items of the
+// photons a wave holds at once, direct-mapped.
+#ifndef VMC_BATCH_LEDGER_SLOTS
+#define VMC_BATCH_LEDGER_SLOTS 4
+#endif
+constexpr int kBatchLedgerSlots = VMC_BATCH_LEDGER_SLOTS;
+
 // Dynamic LDS of one walk workgroup, in bytes, in this order (each part
 // 16-byte aligned):
 //   sector cache values  [S][8W]  u64 (FP64 sums or fixed point)
