@@ -704,16 +704,17 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   int item = 0, tbase = BATCH ? 0 : P.items[0].tbase;
   R kf = BATCH ? R(0) : R(P.items[0].kfac);
 
-  // lanes whose photon's item holds its ledger slot (wave-uniform mask,
-  // updated where slots and photons change hands: at refills)
-  uint64_t own_mask = 0;
+  // whether the lane's photon's item holds its ledger slot (updated where
+  // slots and photons change hands: at refills; a per-lane flag, which the
+  // compiler keeps as a lane mask in scalar registers)
+  bool own_slot = false;
   // ledger word q (0 absorbed, 1 exited, 2 killed, 3 gained, 4 lost) of
   // the lane's photon's item
   // (address-space-typed pointers: the LDS and global adds stay ds_ and
   // global_ atomics instead of merging into one flat atomic)
   auto led_add = [&](int q, double v) {
     if constexpr (BATCH) {
-      if (!((own_mask >> lane) & 1ull)) {
+      if (!own_slot) {
         auto *g = (__attribute__((address_space(1))) double *)(P.ledger + (size_t)item * 5 + q);
         __hip_atomic_fetch_add(g, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
@@ -794,7 +795,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
                 flush_led_slot(sl);
                 wl.set_item(sl, fresh_item);
               }
-              own_mask &= ~__ballot(item == held);
+              own_slot = own_slot && item != held;
             }
           }
         } else {
@@ -826,9 +827,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
         }
         if constexpr (BATCH) {
           // a new photon adds in LDS when its item holds the slot
-          const bool born = alive && !was_alive;
-          const bool mine = born && wl.item_of(item & (kLed - 1)) == item;
-          own_mask = (own_mask & ~__ballot(born)) | __ballot(mine);
+          if (alive && !was_alive) own_slot = wl.item_of(item & (kLed - 1)) == item;
         }
       }
     }
