@@ -490,8 +490,8 @@ __device__ __forceinline__ void create_photon(Photon<R> &p, WalkRng &rs, const I
 // Per-wave LDS ledger of the walk, words in the ledger's order (0 absorbed,
 // 1 exited, 2 killed, 3 gained, 4 lost) plus the wave's loss count.  One
 // problem: a single slot whose absorbed word stays in the lanes (summed
-// across the wave at the end).  A batch: kSlots slots direct-mapped by item,
-// each with the item it sums for (-1: none).
+// across the wave at the end).  A batch: kBatchLedgerSlots slots
+// direct-mapped by item, each with the item it sums for (-1: none).
 template <bool BATCH> struct WaveLedger;
 template <> struct WaveLedger<false> {
   static constexpr int slots = 1, first = 1;
@@ -570,16 +570,16 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   WalkRng rs{0u, 0u, 0u, 0u};   // the walk's random stream of the lane's photon
   uint32_t nrecheck = 0;        // FP32 exit-face choices decided in FP64
   bool alive = false;
-  // absorbed weight accumulates per lane; the rare end-of-photon sums go to
-  // per-wave LDS slots [absorbed, exited, killed, gained, lost] (the ledger's
-  // order) and a per-wave loss count.  A batch keeps one ledger per item
-  // (P.ledger[item * 5 + q]) and each wave kLed slots, direct-mapped by
-  // item: a wave's pool takes the slot of its item when it moves to it,
-  // handing the sums of the item it held there to that item's ledger, so
-  // the photons still alive from the wave's last few pools (other items)
-  // keep adding in LDS; an event whose item lost its slot goes straight to
-  // the item's ledger.  A lane's absorbed weight leaves for its photon's item
-  // when the lane takes a photon of another item, and at the end.
+  // Weight ledger: absorbed weight accumulates per lane; the rare
+  // end-of-photon and roulette sums go to the wave's LDS ledger (WaveLedger).
+  // A batch keeps one ledger per item (P.ledger[item * 5 + q]) and each wave
+  // kLed slots direct-mapped by item: a wave's pool takes the slot of its
+  // item when it moves to it, handing the sums of the item that held it to
+  // that item's ledger, so the photons still alive from the wave's last few
+  // pools (other items) keep adding in LDS; an event whose item lost its
+  // slot goes straight to the item's ledger.  A lane's absorbed weight
+  // leaves for its photon's item when the lane takes a photon of another
+  // item, and at the end.
   double l_abs = 0;
   constexpr int kLed = WaveLedger<BATCH>::slots;
   __shared__ WaveLedger<BATCH> s_wl[MaxBlock<R>::value / 64];

@@ -93,12 +93,10 @@ struct alignas(16) SrcRec {
 // tally words, one 64-byte line of the replica in continuous-wave mode.
 constexpr int kSectorElems = 8;
 
-// Ledger slots per wave of a batch walk (a power of 2): the items of the
-// photons a wave holds at once, direct-mapped.
-#ifndef VMC_BATCH_LEDGER_SLOTS
-#define VMC_BATCH_LEDGER_SLOTS 4
-#endif
-constexpr int kBatchLedgerSlots = VMC_BATCH_LEDGER_SLOTS;
+// Ledger slots per wave of a batch walk: the items of the photons a wave
+// holds at once, direct-mapped (item & (slots - 1)).
+constexpr int kBatchLedgerSlots = 4;
+static_assert((kBatchLedgerSlots & (kBatchLedgerSlots - 1)) == 0, "ledger slots: a power of 2");
 
 // Dynamic LDS of one walk workgroup, in bytes, in this order (each part
 // 16-byte aligned):
