@@ -150,6 +150,10 @@ COUNTER_PROFILES = {("box", "fp32"): "walk3d_fp32_box384k_counters.txt", ("fd", 
 # ones), atomics at any footprint up to 192 MiB
 READ_CEILING = {("box", "fp32"): 61.44e9, ("box", "fp64"): 53.49e9, ("fd", "fp32"): 53.49e9}
 ATOMIC_CEILING = 23.5e9
+# reads and atomics at once, in the walk's proportions (the probe's mixed
+# mode): the box's 0.38 atomic lines per read over 64 MiB of records; the FD
+# box's 0.55 per read over 256 MiB, between the probe's 0.5 and 0.625 rows
+MIXED_CEILING = {("box", "fp32"): (48.36e9, 18.14e9), ("fd", "fp32"): (32.5e9, 17.8e9)}
 
 
 def memory_side(workload, precision, steps_per_s):
@@ -174,11 +178,16 @@ def memory_side(workload, precision, steps_per_s):
         return None
     if rd is None or at is None:
         return None
-    return {"read_requests_per_step": rd, "atomic_requests_per_step": at,
-            "read_g_per_s": round(rd * steps_per_s / 1e9, 2), "atomic_g_per_s": round(at * steps_per_s / 1e9, 2),
-            "read_frac_of_probe": round(rd * steps_per_s / READ_CEILING[key], 3),
-            "atomic_frac_of_probe": round(at * steps_per_s / ATOMIC_CEILING, 3),
-            "source": f"profiles/{name}, profiles/line_probe_mi355x.txt"}
+    out = {"read_requests_per_step": rd, "atomic_requests_per_step": at,
+           "read_g_per_s": round(rd * steps_per_s / 1e9, 2), "atomic_g_per_s": round(at * steps_per_s / 1e9, 2),
+           "read_frac_of_probe": round(rd * steps_per_s / READ_CEILING[key], 3),
+           "atomic_frac_of_probe": round(at * steps_per_s / ATOMIC_CEILING, 3),
+           "source": f"profiles/{name}, profiles/line_probe_mi355x.txt"}
+    if key in MIXED_CEILING:
+        mr, ma = MIXED_CEILING[key]
+        out["read_frac_of_mixed_probe"] = round(rd * steps_per_s / mr, 3)
+        out["atomic_frac_of_mixed_probe"] = round(at * steps_per_s / ma, 3)
+    return out
 
 
 def flops_per_step(workload="box"):

@@ -7,7 +7,9 @@
 // per CU), how many such random 128-byte lines per second the memory system
 // delivers for footprints from L2 size to beyond the Infinity Cache, and how
 // many 64-byte FP64 atomic line requests (eight lanes per line, as the
-// coalesced tally flush issues them) it accepts.  It prints one JSON line
+// coalesced tally flush issues them) it accepts, alone and mixed with the
+// line reads in the walk's proportions (the reads over a record-sized
+// footprint, the atomics over a tally-sized one).  It prints one JSON line
 // per (mode, footprint).  Not part of the library; built beside the command
 // line executables and run by `bench.py --line-probe`.
 #include <hip/hip_runtime.h>
@@ -50,6 +52,28 @@ __global__ void atomic_lines(double *buf, uint32_t nlines, int iters) {
     s = mix32(s + (uint32_t)i);
     __hip_atomic_fetch_add(buf + (size_t)(s % nlines) * 8 + j, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+}
+
+// both at once: every lane reads one random 128-byte line per iteration
+// and its group of eight lanes adds to `k` random 64-byte lines of a
+// second buffer, i.e. k/8 atomic line requests per line read
+__global__ void mixed_lines(const f4 *rbuf, uint32_t rlines, double *abuf, uint32_t alines, int iters, int k,
+                            float *sink) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t grp = t >> 3, j = t & 7u;
+  f4 acc = {0, 0, 0, 0};
+  uint32_t s = mix32(t * 2654435761u + 1u), u = mix32(grp * 2654435761u + 7u);
+  for (int i = 0; i < iters; ++i) {
+    s = mix32(s + (uint32_t)i);
+    const f4 *p = rbuf + (size_t)(s % rlines) * 8;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc += p[q];
+    for (int a = 0; a < k; ++a) {
+      u = mix32(u + (uint32_t)(i * k + a));
+      __hip_atomic_fetch_add(abuf + (size_t)(u % alines) * 8 + j, 1.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (acc.x == -1.0f) sink[t] = acc.y + acc.z + acc.w;   // never true: keeps the loads
 }
 
 __global__ void fill(f4 *buf, size_t n) {
@@ -109,6 +133,36 @@ int main() {
       fflush(stdout);
     }
   }
+  // mixed: the box walk's proportions (0.651 line reads and 0.248 atomic
+  // lines per step: k = 3, 0.375 per read) over its record and tally
+  // footprints, and the FD box walk's (0.792 and 0.436: k = 4 and 5, 0.5
+  // and 0.625 per read) over 256 MiB of records and a 48 MiB tally
+  struct Mix { size_t rsz, asz; int k; };
+  const Mix mixes[] = {{(size_t)64 << 20, (size_t)32 << 20, 3}, {(size_t)256 << 20, (size_t)48 << 20, 4},
+                       {(size_t)256 << 20, (size_t)48 << 20, 5}};
+  double *abuf = nullptr;
+  const size_t acap = (size_t)48 << 20;
+  check(hipMalloc((void **)&abuf, acap), "hipMalloc");
+  for (const Mix &m : mixes) {
+    const uint32_t rlines = (uint32_t)(m.rsz / 128), alines = (uint32_t)(m.asz / 64);
+    const int iters = 400;
+    float ms = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {
+      hipLaunchKernelGGL(fill, dim3(4096), dim3(256), 0, 0, reinterpret_cast<f4 *>(abuf), m.asz / sizeof(f4));
+      check(hipEventRecord(e0), "event");
+      hipLaunchKernelGGL(mixed_lines, dim3(grid), dim3(block), 0, 0, buf, rlines, abuf, alines, iters, m.k, sink);
+      check(hipEventRecord(e1), "event");
+      check(hipEventSynchronize(e1), "kernel");
+      check(hipEventElapsedTime(&ms, e0, e1), "event");
+    }
+    const double lanes = (double)grid * block;
+    const double rl = lanes * iters, al = lanes / 8 * iters * m.k;
+    printf("{\"probe\": \"mixed: random 128-byte line reads over %zu MiB + %d/8 random 64-byte FP64 atomic lines "
+           "per read over %zu MiB\", \"ms\": %.3f, \"g_read_lines_per_s\": %.2f, \"g_atomic_lines_per_s\": %.2f}\n",
+           m.rsz >> 20, m.k, m.asz >> 20, ms, rl / ms / 1e6, al / ms / 1e6);
+    fflush(stdout);
+  }
+  check(hipFree(abuf), "hipFree");
   check(hipFree(buf), "hipFree");
   check(hipFree(sink), "hipFree");
   return 0;
