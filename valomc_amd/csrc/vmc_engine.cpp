@@ -657,16 +657,15 @@ int vmc_problem_sources(const vmc_problem_desc *d, int64_t *faces, int64_t *moth
 int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) try {
   if (!d || !out) return VMC_ERR_INVALID_ARGUMENT;
   *out = nullptr;
-  vmc_problem *m = new vmc_problem();
+  // owned here until it is handed out, so a host allocation failure (which
+  // leaves through the catch below) frees it
+  std::unique_ptr<vmc_problem> own(new vmc_problem());
+  vmc_problem *m = own.get();
   const int st = host_prepare(d, m);
-  if (st != VMC_OK) {
-    delete m;
-    return st;
-  }
+  if (st != VMC_OK) return st;
   try {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-      delete m;
       g_last_error = "no HIP device";
       return VMC_ERR_NO_DEVICE;
     }
@@ -685,10 +684,10 @@ int vmc_problem_create(const vmc_problem_desc *d, vmc_problem **out) try {
     hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
   } catch (HipFail f) {
     const int err = fail_hip(f);
-    vmc_problem_destroy(m);
+    vmc_problem_destroy(own.release());
     return err;
   }
-  *out = m;
+  *out = own.release();
   return VMC_OK;
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
@@ -705,7 +704,8 @@ int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *
     med->g = d->g ? std::vector<double>(d->g, d->g + base->Ne) : base->med->g;
     med->n = d->n ? std::vector<double>(d->n, d->n + base->Ne) : base->med->n;
   }
-  vmc_problem *m = new vmc_problem(base->md, med);
+  std::unique_ptr<vmc_problem> own(new vmc_problem(base->md, med));   // freed on any early return
+  vmc_problem *m = own.get();
   m->dim = base->dim;
   m->nv = base->nv;
   m->nbv = base->nbv;
@@ -740,7 +740,6 @@ int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *
     build_source_records(*m);
     build_boundary(*m);
   } catch (McError e) {
-    delete m;
     return e.code + 1;
   }
   try {
@@ -754,10 +753,10 @@ int vmc_problem_create_variant(const vmc_problem *base, const vmc_variant_desc *
     hip_check(hipMemcpy(m->dbnd.p, m->bnd.data(), m->bnd.size() * sizeof(BndRec), hipMemcpyHostToDevice), "upload");
   } catch (HipFail f) {
     int st = fail_hip(f);
-    vmc_problem_destroy(m);
+    vmc_problem_destroy(own.release());
     return st;
   }
-  *out = m;
+  *out = own.release();
   return VMC_OK;
 } catch (const std::bad_alloc &) {
   return host_out_of_memory();
