@@ -901,7 +901,10 @@ int vmc_batch_run(vmc_batch_item *items, int32_t n, vmc_stats *stats) try {
       arena_collect(g_batch_arena, g.c, raw_e, raw_b, led, cnt, loss, item_led);
       finish_group(items, g, raw_e, raw_b, led, cnt, loss, item_led, &tot);
       tot.grid_blocks = grid;
-      if (g.c.fixed && cnt[4]) range_bad = true;
+      if (g.c.fixed && cnt[4]) {   // the launch's items left the fixed-point range
+        range_bad = true;
+        for (int32_t i : g.idx) items[i].status = VMC_ERR_TALLY_RANGE;
+      }
     }
   } catch (HipFail f) {
     return fail_hip(f);
@@ -1028,7 +1031,10 @@ int vmc_batch_plan_run(vmc_batch_plan *pl, vmc_batch_item *items, int32_t n, vmc
       check_dispenser(cnt[0], pg->w);
       finish_group(items, pg->g, raw_e, raw_b, led, cnt, loss, item_led, &tot);
       tot.grid_blocks = pg->w.grid;
-      if (pg->g.c.fixed && cnt[4]) range_bad = true;
+      if (pg->g.c.fixed && cnt[4]) {
+        range_bad = true;
+        for (int32_t i : pg->g.idx) items[i].status = VMC_ERR_TALLY_RANGE;
+      }
     }
   } catch (HipFail f) {
     return fail_hip(f);
