@@ -94,6 +94,33 @@ def test_tiny_photon_counts_match_oracle(vmc, name, nphot):
     assert res.loss == ref.loss
 
 
+def test_fixed_point_range_is_reported(vmc):
+    """Fixed-point words are exact while a flushed sum stays below 2^30
+    weight units.  A cube 1e10 mm across, non-scattering (mus = 0), index
+    matched (n = 1) and non-absorbing (mua = 0), makes each tet crossing
+    deposit w * ds ~ 1e10 (the reference's non-absorbing tally), so the run
+    ends with VMC_ERR_TALLY_RANGE, alone and in a batch; the same inputs
+    with FP64 tallies are fine."""
+    p = _golden("p3_single_cube")
+    p.r = np.asarray(p.r, dtype=np.float64) * 1e10
+    p.mua = np.zeros_like(p.mua)
+    p.mus = np.zeros_like(p.mus)
+    p.n = np.ones_like(p.n)
+    with vmc.Problem(p) as prob:
+        ok = prob.run(2000, seed=3, precision=vmc.FP64)
+        assert np.all(np.isfinite(ok.ER)) and ok.ER.max() > 0 and ok.loss == 0
+        for prec in (vmc.FP64, vmc.FP32):
+            with pytest.raises(vmc.VmcError) as e:
+                prob.run(2000, seed=3, precision=prec, tally=vmc.TALLY_FIXED64)
+            assert e.value.status == vmc.STATUS_TALLY_RANGE
+        q = _golden("p3_pencil_sphere")
+        with vmc.Problem(q) as fine:
+            items = [(fine, dict(nphoton=2000, seed=1)), (prob, dict(nphoton=2000, seed=3))]
+            with pytest.raises(vmc.VmcError) as e:
+                vmc.batch_run(items, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
+            assert e.value.status == vmc.STATUS_TALLY_RANGE
+
+
 @pytest.mark.parametrize("name", sorted(cases.CASES))
 def test_ledger_closes(vmc, name):
     p = _golden(name)
