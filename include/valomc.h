@@ -208,6 +208,21 @@ int vmc_problem_autotune(vmc_problem *prob, const vmc_run_desc *run, vmc_run_des
    light-source faces. */
 int vmc_problem_check(const vmc_problem_desc *desc, int64_t *HN_out, int64_t *nsources);
 
+/* BuildNeighbourhoods() alone (MC3D.hpp:835-871, MC2D.hpp:724-773), as
+   createBH3mex (cpp/3d/createBH3mex.cpp:26-32) and the classes' public
+   BuildNeighbourhoods() use it: the neighbour table of topology H
+   [Ne*(dim+1)] (0-based node ids below Nr) with the optional boundary
+   entries BH [Nb*dim] (Nb may be 0, BH then NULL), nothing else of
+   ErrorChecks() or Init().  Face f of element e gets the element sharing
+   it, -1-ib for boundary entry ib, or INT64_MAX (the reference's
+   INT_FAST64_MAX) when nothing matches -- in 3D; in 2D an unmatched edge is
+   MISSING_BOUNDARY, as in the reference.  Writes HN_out [Ne*(dim+1)],
+   column-major.  Status: 0, 1 + mcerror (INCONSISTENT_H / INCONSISTENT_BH
+   for a node id outside [0, Nr), the reference's duplicate-neighbour
+   errors), VMC_ERR_INVALID_ARGUMENT for sizes. */
+int vmc_build_neighbours(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *BH,
+                         int64_t *HN_out);
+
 /* Host-only light-source list of Init() (MC3D::BuildLightSource,
    MC3D.hpp:941-1002; MC2D.hpp:851-911): the boundary faces that emit
    light in the reference's element-major order, the element each belongs

@@ -64,6 +64,19 @@ template <class E> static void common(E &mc) {
   printf("copied_seed %lu %d\n", cl.seed, cl.UnifClosed() == (double)std::mt19937(77)() * (1.0 / 4294967295.0));
 }
 
+// BuildNeighbourhoods() on topology alone, as createBH3mex calls it (H and
+// a node array of the right length, nothing else; with_bh: BH too)
+template <class E> static void topology_only(const E &mc, int64_t Nr, bool with_bh, const char *key) {
+  E t;
+  t.H = mc.H;
+  if (with_bh) t.BH = mc.BH;
+  t.r.resize(Nr);
+  t.BuildNeighbourhoods();
+  printf("%s", key);
+  for (int64_t i = 0; i < t.HN.N; ++i) printf(" %lld", (long long)t.HN[i]);
+  printf("\n");
+}
+
 int main(int argc, char **argv) {
   if (argc != 6) return 2;
   const int dim = atoi(argv[2]);
@@ -72,6 +85,8 @@ int main(int argc, char **argv) {
     MC3D mc;
     fill(mc, argv[1], 3, Ne, Nb, Nr);
     common(mc);
+    topology_only(mc, Nr, false, "hn_topology");
+    topology_only(mc, Nr, true, "hn_with_bh");
     printf("volume");
     for (int64_t e = 0; e < Ne; ++e) printf(" %.17g", mc.ElementVolume(e));
     printf("\nbarea");
@@ -97,6 +112,7 @@ int main(int argc, char **argv) {
     MC2D mc;
     fill(mc, argv[1], 2, Ne, Nb, Nr);
     common(mc);
+    topology_only(mc, Nr, true, "hn_with_bh");
     printf("volume");
     for (int64_t e = 0; e < Ne; ++e) printf(" %.17g", mc.ElementArea(e));
     printf("\nbarea");

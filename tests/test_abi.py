@@ -128,6 +128,55 @@ def test_host_prepass_error_status_matches_reference(name):
     assert st == int(d["out_status"])
 
 
+def _neighbours_only(dim, H, BH, Nr):
+    lib = vmc.load_library()
+    lib.vmc_build_neighbours.argtypes = [C.c_int32, C.c_int64, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+    H = np.asfortranarray(np.asarray(H, dtype=np.int64))
+    ne = H.shape[0]
+    bh = None if BH is None else np.asfortranarray(np.asarray(BH, dtype=np.int64))
+    nb = 0 if bh is None else bh.shape[0]
+    hn = np.zeros(ne * (dim + 1), dtype=np.int64)
+    st = lib.vmc_build_neighbours(dim, ne, nb, Nr, H.ctypes.data, None if bh is None else bh.ctypes.data,
+                                  hn.ctypes.data)
+    return st, hn.reshape(dim + 1, ne).T
+
+
+@pytest.mark.parametrize("name", sorted(cases.CASES))
+def test_build_neighbours_alone_matches_reference(name):
+    """vmc_build_neighbours, BuildNeighbourhoods() without the rest of
+    ErrorChecks()/Init(): with BH it writes the reference's table; without
+    BH (createBH3mex's use) a 3D boundary face matches nothing and stays
+    INT64_MAX while interior faces keep their neighbours, and a 2D edge
+    without a neighbour is the reference's MISSING_BOUNDARY."""
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    p = cases.from_arrays(d)
+    ref = d["out_T1_HN"]
+    st, hn = _neighbours_only(p.dim, p.H, p.BH, p.Nr)
+    assert st == 0 and np.array_equal(hn, ref)
+    st, hn = _neighbours_only(p.dim, p.H, None, p.Nr)
+    if p.dim == 3:
+        assert st == 0
+        assert np.array_equal(hn, np.where(ref < 0, np.iinfo(np.int64).max, ref))
+    else:
+        assert st == 1 + vmc.MCERRORS.index("MISSING_BOUNDARY")
+
+
+def test_build_neighbours_alone_errors():
+    d = dict(np.load(os.path.join(GOLDEN, "err_duplicate_neighbour_3d.npz"), allow_pickle=False))
+    p = cases.from_arrays(d)
+    st, _ = _neighbours_only(3, p.H, p.BH, p.Nr)
+    assert st == int(d["out_status"])   # the reference's duplicate-neighbour error
+    p = cases.p3_single_cube()
+    H = np.asarray(p.H).copy()
+    H[0, 0] = p.Nr   # a node id past the node count: INCONSISTENT_H
+    assert _neighbours_only(3, H, p.BH, p.Nr)[0] == 1 + vmc.MCERRORS.index("INCONSISTENT_H")
+    BH = np.asarray(p.BH).copy()
+    BH[0, 0] = -1
+    assert _neighbours_only(3, p.H, BH, p.Nr)[0] == 1 + vmc.MCERRORS.index("INCONSISTENT_BH")
+    assert _neighbours_only(4, p.H, None, p.Nr)[0] == vmc.STATUS_INVALID_ARGUMENT
+
+
 def test_undefined_direction_is_rejected():
     p = cases.p3_given_L_I()
     p.BCLDirType = None          # 'L' and 'I' then have no direction (MC3D.hpp:1112-1192)

@@ -6,9 +6,11 @@ the members the reference's C++ users read: the light-source list
 (LightSources, LightSourcesMother, LightSourcesCDF, MC3D.hpp:155-157,
 941-1002), the random stream (InitRand and the four uniform maps,
 MC3D.hpp:276-303), the sharing assignment (MC3D.hpp:200-273) and the
-geometry queries (MC3D.hpp:306-457, MC2D.hpp:303-390).  They are checked
-against an element-major scan of the reference-written neighbour table,
-numpy cross products and std::mt19937.
+geometry queries (MC3D.hpp:306-457, MC2D.hpp:303-390) and
+BuildNeighbourhoods() on a topology alone (MC3D.hpp:835-871, as
+createBH3mex calls it).  They are checked against an element-major scan
+of the reference-written neighbour table, numpy cross products and
+std::mt19937.
 """
 import os
 import subprocess
@@ -82,6 +84,14 @@ def test_class_members(exe, tmp_path, name):
         size = np.linalg.norm(tri[:, 1] - tri[:, 0], axis=1)
     w = np.cumsum(size * inten[faces])
     assert np.allclose([float(x) for x in out["cdf"]], w / w[-1], rtol=1e-12, atol=0)
+
+    # BuildNeighbourhoods() on topology alone (createBH3mex's use): the
+    # reference's table; without BH the boundary faces stay INT_FAST64_MAX
+    hn_cm = np.asfortranarray(hn).ravel(order="F")
+    assert [int(x) for x in out["hn_with_bh"]] == hn_cm.tolist()
+    if dim == 3:
+        expect = np.where(hn_cm < 0, np.iinfo(np.int64).max, hn_cm)
+        assert [int(x) for x in out["hn_topology"]] == expect.tolist()
 
     # host random stream, assignment
     assert out["rng_mismatch"] == ["0"]

@@ -1,5 +1,6 @@
-"""The MEX gateways (valomc_amd/host/MC3Dmex_gpu.cpp, MC2Dmex_gpu.cpp) run
-without MATLAB against a mock of the MEX API (tests/mex_mock/mex.h): a small
+"""The MEX gateways (valomc_amd/host/MC3Dmex_gpu.cpp, MC2Dmex_gpu.cpp,
+createBH3mex_gpu.cpp) run without MATLAB against a mock of the MEX API
+(tests/mex_mock/mex.h): a small
 driver feeds each gateway the right-hand sides ValoMC.m passes, with their
 MATLAB classes (int64 meshes, int8 boundary types, int64 photon count,
 uint64 [seed, flag]), and writes back what mexFunction returns.
@@ -47,6 +48,20 @@ def drivers(tmp_path_factory):
                                "-Wl,-rpath," + os.path.join(ROOT, "valomc_amd")])
         exe[dim] = out
     return exe
+
+
+@pytest.fixture(scope="module")
+def bh3_driver(tmp_path_factory):
+    import valomc_amd
+    valomc_amd.load_library()
+    out = str(tmp_path_factory.mktemp("mexbh") / "drvbh3")
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           "-I", os.path.join(ROOT, "tests", "mex_mock"),
+                           os.path.join(ROOT, "tests", "mex_mock", "driver.cpp"),
+                           os.path.join(ROOT, "valomc_amd", "host", "createBH3mex_gpu.cpp"), "-o", out,
+                           "-L" + os.path.join(ROOT, "valomc_amd"), "-lvalomc",
+                           "-Wl,-rpath," + os.path.join(ROOT, "valomc_amd")])
+    return out
 
 
 def _rhs(p, nphoton, seed, disable_pbar=1, overrides=None):
@@ -195,3 +210,53 @@ def test_gateway_cancel_keeps_planned_normalisation(drivers, tmp_path):
     # the tallies are still divided by the planned N
     ratio = part[0].real.sum() / full[0].real.sum()
     assert 0.2 < ratio < 0.3, ratio
+
+
+FACES3 = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]   # MC3D.hpp:1013, createBH3mex.cpp:50-77
+
+
+@pytest.mark.parametrize("name", ["p3_pencil_sphere", "p3_cosinic_hetero_n", "p3_single_cube", "p3_iso_mirror"])
+def test_create_bh3_gateway_lists_unshared_faces(bh3_driver, tmp_path, name):
+    """createBH3mex_gpu: [BH] = createBH3mex(H) with 1-based int64 H gives
+    the faces no other tetrahedron shares, 1-based, element-major in the
+    reference's face and vertex order -- here the faces the reference's own
+    neighbour table (written with the golden's BH) sends to the boundary,
+    in the order of its element-major scan."""
+    d = dict(np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False))
+    p = cases.from_arrays(d)
+    H = np.asarray(p.H, dtype=np.int64)
+    hn = d["out_T1_HN"]
+    expect = [[H[e, v] + 1 for v in FACES3[f]] for e in range(p.Ne) for f in range(4) if hn[e, f] < 0]
+    rc, out, err, _, _ = _call(bh3_driver, [("int64", H + 1)], str(tmp_path), nlhs=1)
+    assert rc == 0, err
+    assert out[0].dtype == np.int64 and out[0].shape == (len(expect), 3)
+    assert out[0].tolist() == expect
+    # the same set as the golden's boundary triangles
+    assert sorted(map(sorted, out[0].tolist())) == sorted(sorted(int(v) + 1 for v in row) for row in p.BH)
+
+
+def test_create_bh3_gateway_on_a_grid_by_brute_force(bh3_driver, tmp_path):
+    """On a 4 x 3 x 3 voxel grid (createGridMesh numbering, six tets per
+    voxel): every face counted once among all tetrahedra is a boundary
+    face, in element-major order, two per voxel side on the box's faces."""
+    from valomc_amd import meshgen as mg
+    _, H, _ = mg.grid_mesh(np.arange(4) + 0.5, np.arange(3) + 0.5, np.arange(3) + 0.5)
+    H = np.asarray(H, dtype=np.int64)
+    count = {}
+    for e in range(H.shape[0]):
+        for f in FACES3:
+            k = frozenset(H[e, list(f)].tolist())
+            count[k] = count.get(k, 0) + 1
+    expect = [[H[e, v] + 1 for v in f] for e in range(H.shape[0]) for f in FACES3
+              if count[frozenset(H[e, list(f)].tolist())] == 1]
+    rc, out, err, _, _ = _call(bh3_driver, [("int64", H + 1)], str(tmp_path), nlhs=1)
+    assert rc == 0, err
+    assert out[0].tolist() == expect and len(expect) == 2 * 2 * (4 * 3 + 4 * 3 + 3 * 3)
+
+
+def test_create_bh3_gateway_checks_arguments(bh3_driver, tmp_path):
+    H = np.asarray(cases.p3_single_cube().H, dtype=np.int64) + 1
+    rc, _, err, _, _ = _call(bh3_driver, [("int64", H), ("int64", H)], str(tmp_path), nlhs=1)
+    assert rc == 3 and err.startswith("Syntax:") and "createBH3mex(H)" in err
+    rc, _, err, _, _ = _call(bh3_driver, [("double", H.astype(float))], str(tmp_path), nlhs=1)
+    assert rc == 3 and "int64" in err

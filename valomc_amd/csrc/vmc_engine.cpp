@@ -638,6 +638,13 @@ int vmc_problem_check(const vmc_problem_desc *d, int64_t *HN_out, int64_t *nsour
   return host_out_of_memory();
 }
 
+int vmc_build_neighbours(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *BH,
+                         int64_t *HN_out) try {
+  return neighbours_only(dim, Ne, Nb, Nr, H, BH, HN_out);
+} catch (const std::bad_alloc &) {
+  return host_out_of_memory();
+}
+
 int vmc_problem_sources(const vmc_problem_desc *d, int64_t *faces, int64_t *mothers, double *cdf,
                         int64_t capacity, int64_t *nsources) try {
   if (!d || !nsources || capacity < 0) return VMC_ERR_INVALID_ARGUMENT;

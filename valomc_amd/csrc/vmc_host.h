@@ -261,6 +261,8 @@ struct LaunchCfg {
   double weight0 = kWeight0, chance = kChance;   // roulette constants of the launch
 };
 
+int neighbours_only(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *BH,
+                    int64_t *HN_out);   // vmc_build_neighbours
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c,
                 bool batch);
 void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb);

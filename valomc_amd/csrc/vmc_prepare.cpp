@@ -813,5 +813,37 @@ int host_prepare(const vmc_problem_desc *d, vmc_problem *m) {
   return VMC_OK;
 }
 
+// BuildNeighbourhoods() alone (MC3D.hpp:835-871, MC2D.hpp:724-773): the
+// neighbour table of H and the optional boundary entries BH, nothing else
+// of ErrorChecks() or Init().
+int neighbours_only(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *BH,
+                    int64_t *HN_out) {
+  if ((dim != 2 && dim != 3) || Ne <= 0 || Nb < 0 || Nr <= 0 || !H || !HN_out || (Nb > 0 && !BH))
+    return VMC_ERR_INVALID_ARGUMENT;
+  if (Ne >= kMaxElements || Nb >= INT32_MAX) return VMC_ERR_INVALID_ARGUMENT;
+  vmc_problem m;
+  m.dim = dim;
+  m.nv = dim + 1;
+  m.nbv = dim;
+  m.Ne = Ne;
+  m.Nb = Nb;
+  m.Nr = Nr;
+  copy_in(m.H, H, Ne * m.nv);
+  if (Nb > 0) copy_in(m.BH, BH, Nb * m.nbv);
+  try {
+    // node ids index the reference's vertex incidence lists (sized by the
+    // node count), so they must lie in [0, Nr)
+    for (int64_t v : m.H)
+      if (v < 0 || v >= Nr) throw McError{INCONSISTENT_H};
+    for (int64_t v : m.BH)
+      if (v < 0 || v >= Nr) throw McError{INCONSISTENT_BH};
+    build_neighbours(m);
+  } catch (McError e) {
+    return e.code + 1;
+  }
+  std::copy(m.HN.begin(), m.HN.end(), HN_out);
+  return VMC_OK;
+}
+
 
 } // namespace vmc

@@ -153,14 +153,15 @@ public:
     }
   }
 
+  // Neighbour table of H (and BH, if set) alone (MC3D.hpp:835-871): faces
+  // that match nothing stay INT_FAST64_MAX, as createBH3mex reads them.
+  // Needs only H, BH and the node count r.Nx; a given HN is kept.
   void BuildNeighbourhoods() {
-    vmc_problem_desc d = desc();
+    if (HN.N == H.N) return;   // given (MC3D.hpp:839)
     std::vector<int64_t> hn((size_t)H.Nx * (DIM + 1));
-    vmc_throw_status(vmc_problem_check(&d, hn.data(), nullptr));
-    if (HN.N != H.N) {
-      HN.resize(H.Nx, DIM + 1);
-      for (size_t i = 0; i < hn.size(); ++i) HN[i] = hn[i];
-    }
+    vmc_throw_status(vmc_build_neighbours(DIM, H.Nx, BH.Nx, r.Nx, H.data, BH.N ? BH.data : nullptr, hn.data()));
+    HN.resize(H.Nx, DIM + 1);
+    for (size_t i = 0; i < hn.size(); ++i) HN[i] = hn[i];
   }
 
   // MonteCarlo (MC3D.hpp:1641-1822): transport on the GPU, normalised
