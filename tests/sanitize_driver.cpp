@@ -77,6 +77,20 @@ int main(int argc, char **argv) {
   printf("bad_h %d\n", vmc_problem_check(&pd, nullptr, nullptr));
   pd.H = H.data();
   {
+    // BuildNeighbourhoods alone, with and without the boundary entries:
+    // without them the boundary faces match nothing (3D: INT64_MAX)
+    std::vector<int64_t> hb(hn.size()), ht(hn.size());
+    const int sb = vmc_build_neighbours(dim, Ne, Nb, Nr, H.data(), pd.BH, hb.data());
+    const int st = vmc_build_neighbours(dim, Ne, 0, Nr, H.data(), nullptr, ht.data());
+    int64_t same = 0, unmatched = 0;
+    for (size_t i = 0; i < hn.size(); ++i) {
+      same += hb[i] == hn[i];
+      unmatched += ht[i] == INT64_MAX;
+    }
+    printf("neighbours_only %d %lld %lld %d %lld\n", sb, (long long)same, (long long)hn.size(), st,
+           (long long)unmatched);
+  }
+  {
     // the device order the pre-pass builds: slots of whole sectors, every
     // element in one slot and back, no neighbour code or light source on a
     // padding slot; and, for a grid mesh whose element e lies in cell

@@ -97,6 +97,12 @@ def _check(out, nphoton, expect_sources=True, grid=False):
         assert int(out["nsources"][0][0]) > 0
         assert out["sources"][0] == ["0", out["nsources"][0][0]]
     assert out["bad_h"] == [["11"]]                      # 1 + INCONSISTENT_H
+    # BuildNeighbourhoods alone: with BH the pre-pass's table; without it a
+    # 3D mesh leaves exactly its boundary faces unmatched and a 2D one is
+    # MISSING_BOUNDARY (1 + 15)
+    sb, same, total, st, unmatched = (int(x) for x in out["neighbours_only"][0])
+    assert sb == 0 and same == total
+    assert st == 16 or (st == 0 and unmatched == int(out["nsources"][0][2]))
     if nphoton >= 0:
         for mode in out["oracle"]:
             assert mode[1] == "0" and abs(float(mode[5])) < 1e-9, mode
