@@ -22,7 +22,7 @@
 #include "valomc.h"
 
 void mexFunction(int nlhs, mxArray **plhs, int nrhs, const mxArray **prhs) {
-  if (nrhs != 1 || nlhs != 1) mexErrMsgTxt("Syntax:\n [HN] = createBH3mex(H)\n");
+  if (nrhs != 1 || nlhs != 1) mexErrMsgTxt("Syntax:\n [BH] = createBH3mex(H)\n");
   const mxArray *h = prhs[0];
   if (mxGetClassID(h) != mxINT64_CLASS || mxGetN(h) != 4 || mxGetM(h) == 0)
     mexErrMsgTxt("createBH3mex: H must be a non-empty int64 matrix with 4 columns");
