@@ -160,7 +160,8 @@ def test_many_items_equal_single_runs(vmc):
     """Two dozen items of one problem, with uneven photon counts (one large
     item, an empty one, a single photon, one just past a dispenser chunk):
     chunks of many items are in flight at once and items end at different
-    times, yet each item equals its solo run bit for bit (fixed point)."""
+    times, yet each item equals its solo run bit for bit (fixed point) and
+    keeps its own weight ledger (24 items share a wave's four ledger slots)."""
     p = _golden("p3_pencil_sphere")
     q = _golden("p3_direct_abs_fd")
     counts = [200000, 0, 1, 129] + [3000 + 1500 * k for k in range(20)]
@@ -170,21 +171,26 @@ def test_many_items_equal_single_runs(vmc):
         print(f"batch of {len(items)} CW items", flush=True)
         res, stats = vmc.batch_run(items, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
         assert stats.photons == sum(counts)
+        assert np.allclose(stats.ledger, np.sum([r.ledger for r in res], axis=0), rtol=1e-12, atol=1e-9)
         for (pr, kw), r in zip(items, res):
             kw = dict(kw)
-            alone = pr.run(kw.pop("nphoton"), precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+            n = kw.pop("nphoton")
+            alone = pr.run(n, precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
             for k in OUT:
                 assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
             assert r.loss == alone.loss
+            _check_ledger(r, alone, n, 1e-6)
         itemsq = [(probq, dict(nphoton=n, seed=5, stream_tag=k, f=q.f, phase0=q.phase0))
                   for k, n in enumerate(counts[4:])]
         print(f"batch of {len(itemsq)} FD items", flush=True)
         resq, _ = vmc.batch_run(itemsq, precision=vmc.FP32, tally=vmc.TALLY_FIXED64)
         for (pr, kw), r in zip(itemsq, resq):
             kw = dict(kw)
-            alone = pr.run(kw.pop("nphoton"), precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+            n = kw.pop("nphoton")
+            alone = pr.run(n, precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
             for k in OUT:
                 assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
+            _check_ledger(r, alone, n, 1e-6)
     finally:
         prob.close()
         probq.close()
