@@ -505,8 +505,12 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if tps is None else round(tps * steps_total / a.steps),
-                "kernel": f"walk_kernel<{p.dim},{'double' if a.precision == 'fp64' else 'float'},"
+                "kernel": f"{'walk_small_kernel' if runs[0].lds_mesh else 'walk_kernel'}"
+                          f"<{p.dim},{'double' if a.precision == 'fp64' else 'float'},"
                           f"{'FD' if fd else 'CW'},{'fixed' if a.tally == 'fixed' else 'fp64'}-tally>",
+                # a small mesh is walked from records held in LDS: the
+                # algorithmic bytes are then served by LDS, not by HBM
+                "records_in_lds": bool(runs[0].lds_mesh),
                 "bytes_per_step": bstep,
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}

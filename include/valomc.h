@@ -148,6 +148,11 @@ typedef struct vmc_stats {
   double ledger[5];
   int64_t rechecks;      /* FP32 exit-face choices re-evaluated in FP64          */
   int32_t grid_blocks, block_size;
+  int32_t lds_mesh;      /* 1: a small mesh whose element records every
+                            workgroup held in LDS (default cache knobs, the
+                            records and a cache slot per 8 elements fit the
+                            workgroup's LDS share); 0: records read from
+                            device memory                                     */
 } vmc_stats;
 
 /* progress callback: percent done; return 0 to abort (MC3D.hpp:1688-1695) */

@@ -234,6 +234,47 @@ def test_tally_paths_agree_bit_for_bit(vmc, name):
                 assert np.array_equal(getattr(r, k), getattr(base, k)), (kw, k)
 
 
+@pytest.mark.parametrize("name", ["p3_single_cube", "p3_direct_abs_fd", "p2_square_cosinic", "p2_iso_gauss_fd",
+                                  "p2_mirror_hetero"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_small_mesh_in_lds_agrees_bit_for_bit(vmc, name, precision):
+    """A mesh whose element records fit a workgroup's LDS share is walked
+    from LDS with a cache slot per sector (reported as lds_mesh); explicit
+    cache knobs keep the records in device memory.  With fixed-point tallies
+    both give the same bits, the same steps and the same losses."""
+    p = _golden(name)
+    p.Nphoton = 30000
+    prec = vmc.FP32 if precision == "fp32" else vmc.FP64
+    print(f"case {name} {precision}", flush=True)
+    with vmc.Problem(p) as prob:
+        run = lambda **kw: prob.run(p.Nphoton, seed=17, f=p.f, phase0=p.phase0, precision=prec,
+                                    tally=vmc.TALLY_FIXED64, **kw)
+        small = run()
+        # 384 FP64 tetrahedra (256-byte records) exceed the 80 KB of a
+        # 256-lane FP64 workgroup; every other case here fits
+        assert small.lds_mesh == (not (precision == "fp64" and p.dim == 3 and p.Ne > 100)), p.Ne
+        for kw in (dict(lds_slots=-1), dict(lds_slots=5), dict(cache_epoch=2)):
+            r = run(**kw)
+            assert not r.lds_mesh
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(small, k)), (kw, k)
+            assert (r.loss, r.steps) == (small.loss, small.steps), kw
+
+
+def test_mesh_residency_follows_size(vmc):
+    """The 200-triangle square of the reference's 2D example is held in LDS
+    in FP32 and FP64; the 1296-tetrahedron golden mesh (166 KB of FP32
+    records) is not."""
+    from valomc_amd import meshgen
+    sq = meshgen.square_2d(nphoton=20000, seed=1)
+    big = _golden("p3_pencil_sphere")
+    with vmc.Problem(sq) as a, vmc.Problem(big) as b:
+        assert a.run(20000, precision=vmc.FP32).lds_mesh
+        assert a.run(20000, precision=vmc.FP64).lds_mesh
+        assert not b.run(20000, precision=vmc.FP32).lds_mesh
+        assert not b.run(20000, precision=vmc.FP64).lds_mesh
+
+
 def test_tally_flush_mode_is_validated(vmc):
     """direct_flush takes 0 or 1 only."""
     p = _golden("p3_pencil_sphere")

@@ -77,6 +77,7 @@ class Stats(C.Structure):
         ("photons", C.c_int64), ("loss", C.c_int64), ("steps", C.c_int64),
         ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("ledger", C.c_double * 5),
         ("rechecks", C.c_int64), ("grid_blocks", C.c_int32), ("block_size", C.c_int32),
+        ("lds_mesh", C.c_int32),
     ]
 
 
@@ -194,6 +195,7 @@ class RunResult:
     block_size: int
     status: int = 0
     rechecks: int = 0
+    lds_mesh: bool = False   # the element records were held in LDS (small mesh)
 
 
 def make_run_desc(nphoton: int, seed: int = 5489, f: float = 0.0, phase0: float = 0.0,
@@ -389,7 +391,8 @@ class Problem:
         return RunResult(status=st, loss=int(s.loss), steps=int(s.steps), kernel_ms=float(s.kernel_ms),
                          total_ms=float(s.total_ms), ledger=np.array(list(s.ledger)),
                          photons=int(s.photons), grid_blocks=int(s.grid_blocks),
-                         block_size=int(s.block_size), rechecks=int(s.rechecks), **out)
+                         block_size=int(s.block_size), rechecks=int(s.rechecks),
+                         lds_mesh=bool(s.lds_mesh), **out)
 
     def autotune(self, nphoton: int, trial_photons: int = 0, **run_kwargs) -> dict:
         """vmc_problem_autotune: the launch shape and tally strategy that run

@@ -40,6 +40,18 @@ int run_model(const vmc_run_desc *run, RunModel *m) {
 // the launch shape on a device with `num_cus` CUs of `lds_per_cu` bytes of
 // LDS; `batch`: the launch may walk several items (their kernels keep four
 // ledger slots per wave in static LDS).
+// LDS of one workgroup when a CU holds the waves its registers allow (FP32
+// 16, FP64 8), and the part of it the lane buffers and the kernel's static
+// LDS take (16 waves' ledgers: 40 bytes a wave for one item, 44 a slot + 8
+// for a batch; plus alignment).
+static void lds_budget(const LaunchCfg &c, int64_t lds_per_cu, bool batch, size_t *per_block, size_t *fixed_part) {
+  const int waves_per_cu = c.f64 ? 8 : 16;
+  const int bpc_target = std::max(1, waves_per_cu / (c.block / 64));
+  *per_block = (size_t)lds_per_cu / bpc_target;
+  const size_t wave_ledger = batch ? ((size_t)kBatchLedgerSlots * 44 + 8 + 7) / 8 * 8 : 40;
+  *fixed_part = WalkLds(c.W, c.f64 ? 8 : 4, 0, c.block).total + 16 * wave_ledger + 384;
+}
+
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c,
                 bool batch) {
   (void)num_cus;
@@ -72,13 +84,8 @@ int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run
   // the workgroups that fit by registers, -1 = off
   c->slots = run->lds_slots < 0 ? 0 : run->lds_slots;
   if (run->lds_slots == 0) {
-    const int waves_per_cu = c->f64 ? 8 : 16;
-    const int bpc_target = std::max(1, waves_per_cu / (c->block / 64));
-    const size_t per_block = (size_t)lds_per_cu / bpc_target;
-    // + the kernel's static LDS (16 waves' ledgers: 40 bytes a wave for one
-    // item, 44 a slot + 8 for a batch) and alignment
-    const size_t wave_ledger = batch ? ((size_t)kBatchLedgerSlots * 44 + 8 + 7) / 8 * 8 : 40;
-    const size_t fixed_part = WalkLds(c->W, rbytes, 0, c->block).total + 16 * wave_ledger + 384;
+    size_t per_block, fixed_part;
+    lds_budget(*c, lds_per_cu, batch, &per_block, &fixed_part);
     const size_t entry = (size_t)kSectorElems * c->W * 8 + 4;
     c->slots = per_block > fixed_part ? (int)((per_block - fixed_part) / entry) : 0;
   }
@@ -115,6 +122,32 @@ void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t 
   const bool single = bytes > 4.0 * (1 << 20) && bytes <= 64.0 * (1 << 20);
   c.xcc = !single;
   c.nrep = single ? 1 : 8;
+}
+
+static size_t elem_bytes(int dim, bool f64) {
+  if (dim == 3) return f64 ? sizeof(Elem<4, double>) : sizeof(Elem<4, float>);
+  return f64 ? sizeof(Elem<3, double>) : sizeof(Elem<3, float>);
+}
+
+// ne: the problem's device element slots (whole sectors).  With the records
+// in LDS the walk reads no record from memory, and with a slot per sector
+// every deposit stays in LDS until the workgroup's final flush.  Explicit
+// cache knobs keep the general path.
+void choose_small(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t lds_per_cu) {
+  c.small = false;
+  if (run->lds_slots != 0 || run->cache_epoch > 0 || ne <= 0 || ne % kSectorElems != 0) return;
+  size_t per_block, fixed_part;
+  lds_budget(c, lds_per_cu, false, &per_block, &fixed_part);
+  const size_t rec = elem_bytes(c.dim, c.f64);
+  if ((size_t)ne * rec >= per_block) return;
+  const int nsect = (int)(ne / kSectorElems);
+  const size_t lay = WalkLds(c.W, c.f64 ? 8 : 4, nsect, c.block).total;
+  const size_t statics = fixed_part - WalkLds(c.W, c.f64 ? 8 : 4, 0, c.block).total;
+  if (lay + (size_t)ne * rec + statics > per_block) return;
+  c.small = true;
+  c.small_elems = (int32_t)ne;
+  c.slots = nsect;
+  c.lds = lay + (size_t)ne * rec;
 }
 
 // Sizes the arena for ne element and nb boundary slots and clears its
@@ -160,7 +193,8 @@ WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<I
   if (v == 0) return w;
   w.batch = items.size() > 1;
   int bpc = 0;
-  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, w.batch, c.block, c.lds, &bpc), "occupancy");
+  if (c.small && w.batch) throw HipFail{hipErrorInvalidValue, "a small-mesh launch walks one item"};
+  hip_check(walk_occupancy(c.dim, c.f64, c.fd, c.fixed, w.batch, c.small, c.block, c.lds, &bpc), "occupancy");
   if (bpc < 1) throw HipFail{hipErrorInvalidConfiguration, "workgroup does not fit on a CU (block size / LDS tally cache)"};
   w.grid = (c.bpc > 0 ? std::min(bpc, c.bpc) : bpc) * num_cus;
   hip_check(hipMemcpyAsync(a.items.p, items.data(), items.size() * sizeof(ItemRec), hipMemcpyHostToDevice, a.stream),
@@ -191,13 +225,15 @@ WalkLaunch prepare_walk(Arena &a, int num_cus, const LaunchCfg &c, std::vector<I
   P.weight0_f = (float)P.weight0;
   P.chance_f = (float)P.chance;
   P.fix_scale = 4294967296.0;
+  P.mesh_elems = c.small ? c.small_elems : 0;
   return w;
 }
 
 void enqueue_walk(Arena &a, const LaunchCfg &c, const WalkLaunch &w) {
   if (w.grid == 0) return;
   hip_check(launch_fill_u64(a.counters.p, 1, 0ull, a.stream), "fill");   // dispenser
-  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, w.batch, w.grid, c.block, c.lds, a.stream, w.P), "walk");
+  hip_check(launch_walk(c.dim, c.f64, c.fd, c.fixed, w.batch, c.small, w.grid, c.block, c.lds, a.stream, w.P),
+            "walk");
 }
 
 void check_dispenser(unsigned long long dispensed, const WalkLaunch &w) {
@@ -539,6 +575,7 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
     hip_check(hipSetDevice(m->device), "hipSetDevice");
     upload_params(*m, c.f64);
     choose_replicas(c, run, ne, nb);
+    choose_small(c, run, ne, m->lds_per_cu);
     Arena view;
     std::unique_lock<std::mutex> turn;
     if (use_scratch) {
@@ -580,6 +617,7 @@ int problem_run(vmc_problem *m, const vmc_run_desc *run, vmc_output *out, vmc_st
     for (int q = 0; q < 5; ++q) stats->ledger[q] = led[q];
     stats->grid_blocks = grid;
     stats->block_size = c.block;
+    stats->lds_mesh = c.small ? 1 : 0;
     stats->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   }

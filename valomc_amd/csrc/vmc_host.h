@@ -22,10 +22,11 @@
 namespace vmc {
 
 // kernel launchers (vmc_kernels.hip)
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, int block, size_t lds_bytes,
-                          int *blocks_per_cu);
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, int grid, int block, size_t lds_bytes,
-                       hipStream_t s, const WalkParams &P);
+// (small: walk_small_kernel, the item's element records in LDS; one item)
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, bool small, int block,
+                          size_t lds_bytes, int *blocks_per_cu);
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, bool small, int grid, int block,
+                       size_t lds_bytes, hipStream_t s, const WalkParams &P);
 hipError_t launch_fill_u64(unsigned long long *p, size_t n, unsigned long long v, hipStream_t s);
 hipError_t launch_fill_ranges(unsigned long long *p, size_t n, int nrep, size_t stride, unsigned long long v,
                               hipStream_t s);
@@ -259,6 +260,10 @@ struct LaunchCfg {
   int bpc = 0;     // workgroups per CU: 0 = as many as fit, else at most this many
   size_t lds = 0;
   double weight0 = kWeight0, chance = kChance;   // roulette constants of the launch
+  // small mesh (choose_small): the single item's small_elems element records
+  // are staged in each workgroup's LDS, with one cache slot per sector
+  bool small = false;
+  int32_t small_elems = 0;
 };
 
 int neighbours_only(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64_t *H, const int64_t *BH,
@@ -266,6 +271,10 @@ int neighbours_only(int32_t dim, int64_t Ne, int64_t Nb, int64_t Nr, const int64
 int resolve_cfg(int dim, int num_cus, int64_t lds_per_cu, bool fd, const vmc_run_desc *run, LaunchCfg *c,
                 bool batch);
 void choose_replicas(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t nb);
+// A single problem of ne elements walks from LDS-resident records when they
+// and a cache slot per sector fit the workgroup's LDS share and the cache
+// knobs are the defaults; sets c.small, c.slots and c.lds.
+void choose_small(LaunchCfg &c, const vmc_run_desc *run, int64_t ne, int64_t lds_per_cu);
 // The arena's buffers for ne element and nb boundary slots and nitems
 // items (no device work; nothing is allocated under stream capture).
 void arena_alloc(Arena &a, const LaunchCfg &c, int64_t ne, int64_t nb, size_t nitems);

@@ -22,6 +22,12 @@
 // photon moves on.  Those go to one of `nrep` replicas, the XCD's own
 // (workgroup-scope atomics) or, for an explicit count, one picked by
 // workgroup index (device scope); the replicas are summed after the walk.
+//
+// Small meshes: when a single problem's element records and one cache slot
+// per sector fit in a workgroup's LDS, walk_small_kernel copies the records
+// into LDS at the start and every sector owns its slot, so the walk reads
+// no element record from memory and adds no tally word to it before the
+// workgroup's final flush.
 #include "vmc_types.h"
 
 #include <type_traits>
@@ -180,8 +186,9 @@ template <> struct Tol<double> { static constexpr double out = 1e-12; };
 // launch; flushes of a claimed sector stay in LDS and reach the global
 // replica once, when the workgroup finishes.  Every photon starts at a light
 // source, so the slots fill with the sectors next to the sources, which
-// take the most flushes.
-template <bool FIXED, int WS> struct SectorCache {
+// take the most flushes.  IDENT (small meshes): there is a slot per sector,
+// slot s holds sector s from the start and nothing is claimed.
+template <bool FIXED, int WS, bool IDENT> struct SectorCache {
   int *tag;                  // [S] sector id or -1
   unsigned long long *val;   // [S][WS] FP64 bits or fixed point
   int nslots;                // 0: cache off
@@ -191,6 +198,7 @@ template <bool FIXED, int WS> struct SectorCache {
   // the sector's slot, claimed if still free; -1 when the slot belongs to
   // another sector or the cache is off
   __device__ __forceinline__ int claim(int sect) {
+    if constexpr (IDENT) return sect;
     if (nslots == 0) return -1;
     const int sl = slot_of(sect);
     int t = __hip_atomic_load(&tag[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -249,13 +257,17 @@ template <class T> __device__ __forceinline__ const __attribute__((address_space
   return (const __attribute__((address_space(1))) T *)p;
 }
 
-// Load the record of element `el` into registers.
-template <int NF, class R>
-__device__ __forceinline__ void load_elem(ElemPtr<NF, R> E, int el,
+// Element records staged in a workgroup's LDS (small meshes).
+template <int NF, class R> using ElemLds = const __attribute__((address_space(3))) Elem<NF, R> *;
+
+// Load the record of element `el` into registers (from global memory or,
+// for a small mesh, from LDS).
+template <int NF, class R, class EP>
+__device__ __forceinline__ void load_elem(EP E, int el,
                                           typename Vec4T<R>::type (&pl)[NF],
                                           typename Vec4T<R>::type &opt, i4 &nb,
                                           typename Vec2T<R>::type (&nbo)[NF]) {
-  ElemPtr<NF, R> e = E + el;
+  const auto e = E + el;
 #pragma unroll
   for (int f = 0; f < NF; ++f) pl[f] = e->pl[f];
   opt = e->opt;
@@ -552,9 +564,11 @@ template <> struct ExitWords<2, true> { static constexpr int n = 4; static const
 // BATCH: items of different problems share the launch, so each lane carries
 // its photon's element records, tally base and wave-number factor; without
 // it every photon belongs to item 0 and these stay in scalar registers.
-template <int D, class R, bool FD, bool FIXED, bool BATCH>
-__global__ void __launch_bounds__(MaxBlock<R>::value)
-    __attribute__((amdgpu_waves_per_eu(MinWaves<R>::value))) walk_kernel(const WalkParams P) {
+// SMALL (one item only): the item's P.mesh_elems element records are read
+// from a copy in LDS and the sector cache has one slot per sector.
+template <int D, class R, bool FD, bool FIXED, bool BATCH, bool SMALL>
+__device__ __forceinline__ void walk_body(const WalkParams &P) {
+  static_assert(!(SMALL && BATCH), "a small-mesh launch walks one item");
   constexpr int NF = D + 1;
   constexpr int W = FD ? 2 : 1;
   typedef ExitWords<D, FD> XW;
@@ -590,12 +604,23 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   extern __shared__ __align__(16) unsigned char s_dyn[];
   const int S = P.lds_slots > 0 ? P.lds_slots : 0;
   const WalkLds lay(W, (int)sizeof(R), S, (int)blockDim.x);
-  SectorCache<FIXED, WS> cache{reinterpret_cast<int *>(s_dyn + lay.ctag),
-                               reinterpret_cast<unsigned long long *>(s_dyn + lay.cval), S};
+  SectorCache<FIXED, WS, SMALL> cache{reinterpret_cast<int *>(s_dyn + lay.ctag),
+                                      reinterpret_cast<unsigned long long *>(s_dyn + lay.cval), S};
   R *const sbuf = reinterpret_cast<R *>(s_dyn + lay.sbuf) + threadIdx.x;   // word j at [j * blockDim]
   const int bstride = (int)blockDim.x;
-  for (int i = threadIdx.x; i < S; i += blockDim.x) cache.tag[i] = -1;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) cache.tag[i] = SMALL ? i : -1;
   for (int i = threadIdx.x; i < S * WS; i += blockDim.x) cache.val[i] = 0ull;
+  // small mesh: the item's element records, copied after the lane buffers
+  // and cache (16-byte words; the records are multiples of 16 bytes)
+  const ElemLds<NF, R> EL = (ElemLds<NF, R>)(s_dyn + lay.total);
+  if constexpr (SMALL) {
+    static_assert(sizeof(Elem<NF, R>) % 16 == 0, "records are copied in 16-byte words");
+    typedef typename Vec4T<float>::type W16;
+    const W16 *src = reinterpret_cast<const W16 *>(P.items[0].elems);
+    auto *dst = (__attribute__((address_space(3))) W16 *)(s_dyn + lay.total);
+    const int n16 = P.mesh_elems * (int)(sizeof(Elem<NF, R>) / 16);
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+  }
 #pragma unroll
   for (int j = 0; j < WS; ++j) sbuf[j * bstride] = R(0);
   // cache epochs: waves that have finished, counted in LDS
@@ -611,7 +636,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   // whichever sectors hold slots.
   int sect = -1;
   int cslot = -1;
-  const int epoch = S > 0 ? P.cache_epoch : 0;   // wave-uniform
+  const int epoch = S > 0 && !SMALL ? P.cache_epoch : 0;   // wave-uniform
   const int nwaves = (int)(blockDim.x >> 6);
   int pass = 0;            // wave-uniform: passes since the last epoch
   bool wave_done = false;  // wave-uniform: no photons left and none to come
@@ -821,7 +846,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
               kf = R(it.kfac);
             }
             create_photon<D, R>(p, rs, it, P);
-            p.prop = -m_log(Unif<R>::open(rs.next(P))) / E[p.el].opt.y;
+            p.prop = -m_log(Unif<R>::open(rs.next(P))) / (SMALL ? EL[p.el].opt.y : E[p.el].opt.y);
             alive = true;
           }
         }
@@ -861,7 +886,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       V4 opt;
       i4 nb;
       typename Vec2T<R>::type nbo[NF];
-      load_elem<NF, R>(E, p.el, pl, opt, nb, nbo);
+      if constexpr (SMALL) load_elem<NF, R>(EL, p.el, pl, opt, nb, nbo);
+      else load_elem<NF, R>(E, p.el, pl, opt, nb, nbo);
       ++p.nsteps;
       // one element visit (entry to exit, scatterings inside included) adds
       // its deposits once
@@ -944,7 +970,8 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
           V4 opt2;
           i4 nb2;
           typename Vec2T<R>::type nbo2[NF];
-          load_elem<NF, R>(E, p.el, pl2, opt2, nb2, nbo2);
+          if constexpr (SMALL) load_elem<NF, R>(EL, p.el, pl2, opt2, nb2, nbo2);
+          else load_elem<NF, R>(E, p.el, pl2, opt2, nb2, nbo2);
           double bt = 0.0;
           int g = -1;
 #pragma unroll
@@ -982,7 +1009,7 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
       // outward unit normal of the exit face, for the rarer boundary and
       // refraction branches (re-read rather than carried through the step)
       auto exit_normal = [&](R (&nrm)[3]) {
-        const V4 q = (E + p.el)->pl[bf];
+        const V4 q = SMALL ? (EL + p.el)->pl[bf] : (E + p.el)->pl[bf];
         nrm[0] = q.x;
         nrm[1] = q.y;
         nrm[2] = D == 3 ? q.z : R(0);
@@ -1233,6 +1260,18 @@ __global__ void __launch_bounds__(MaxBlock<R>::value)
   }
 }
 
+template <int D, class R, bool FD, bool FIXED, bool BATCH>
+__global__ void __launch_bounds__(MaxBlock<R>::value)
+    __attribute__((amdgpu_waves_per_eu(MinWaves<R>::value))) walk_kernel(const WalkParams P) {
+  walk_body<D, R, FD, FIXED, BATCH, false>(P);
+}
+
+template <int D, class R, bool FD, bool FIXED>
+__global__ void __launch_bounds__(MaxBlock<R>::value)
+    __attribute__((amdgpu_waves_per_eu(MinWaves<R>::value))) walk_small_kernel(const WalkParams P) {
+  walk_body<D, R, FD, FIXED, false, true>(P);
+}
+
 // ---------------------------------------------------------------------------
 // Helper kernels
 // ---------------------------------------------------------------------------
@@ -1309,7 +1348,10 @@ hipError_t scratch_address(int device, void **ptr) {
 typedef void (*WalkFn)(const WalkParams);
 
 template <int D, class R>
-static WalkFn walk_fn_dr(bool fd, bool fixed, bool batch) {
+static WalkFn walk_fn_dr(bool fd, bool fixed, bool batch, bool small) {
+  if (small)
+    return fd ? (fixed ? walk_small_kernel<D, R, true, true> : walk_small_kernel<D, R, true, false>)
+              : (fixed ? walk_small_kernel<D, R, false, true> : walk_small_kernel<D, R, false, false>);
   if (batch)
     return fd ? (fixed ? walk_kernel<D, R, true, true, true> : walk_kernel<D, R, true, false, true>)
               : (fixed ? walk_kernel<D, R, false, true, true> : walk_kernel<D, R, false, false, true>);
@@ -1317,20 +1359,23 @@ static WalkFn walk_fn_dr(bool fd, bool fixed, bool batch) {
             : (fixed ? walk_kernel<D, R, false, true, false> : walk_kernel<D, R, false, false, false>);
 }
 
-static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed, bool batch) {
-  if (dim == 3) return f64 ? walk_fn_dr<3, double>(fd, fixed, batch) : walk_fn_dr<3, float>(fd, fixed, batch);
-  return f64 ? walk_fn_dr<2, double>(fd, fixed, batch) : walk_fn_dr<2, float>(fd, fixed, batch);
+static WalkFn walk_fn(int dim, bool f64, bool fd, bool fixed, bool batch, bool small) {
+  if (dim == 3)
+    return f64 ? walk_fn_dr<3, double>(fd, fixed, batch, small) : walk_fn_dr<3, float>(fd, fixed, batch, small);
+  return f64 ? walk_fn_dr<2, double>(fd, fixed, batch, small) : walk_fn_dr<2, float>(fd, fixed, batch, small);
 }
 
-hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, int block, size_t lds_bytes,
-                          int *blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed, batch), block,
-                                                      lds_bytes);
+hipError_t walk_occupancy(int dim, bool f64, bool fd, bool fixed, bool batch, bool small, int block,
+                          size_t lds_bytes, int *blocks_per_cu) {
+  if (batch && small) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn(dim, f64, fd, fixed, batch, small),
+                                                      block, lds_bytes);
 }
 
-hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, int grid, int block, size_t lds_bytes,
-                       hipStream_t s, const WalkParams &P) {
-  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed, batch), dim3(grid), dim3(block), lds_bytes, s, P);
+hipError_t launch_walk(int dim, bool f64, bool fd, bool fixed, bool batch, bool small, int grid, int block,
+                       size_t lds_bytes, hipStream_t s, const WalkParams &P) {
+  if (batch && small) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(walk_fn(dim, f64, fd, fixed, batch, small), dim3(grid), dim3(block), lds_bytes, s, P);
   return hipGetLastError();
 }
 

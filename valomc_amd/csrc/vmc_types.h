@@ -141,6 +141,7 @@ struct alignas(16) ItemRec {
 struct WalkParams {
   const ItemRec *items;
   int32_t nitems;
+  int32_t mesh_elems;           // small-mesh kernel: element records staged in LDS
   uint64_t vtotal;              // virtual photon ids [0, vtotal) of this launch
   void *etally;
   void *btally;
