@@ -149,6 +149,9 @@ __device__ __forceinline__ float w_expm1_neg(float x) {
   return x > -0.0625f ? p : e;
 }
 __device__ __forceinline__ double w_expm1_neg(double x) { return expm1(x); }
+// sin and cos of pi x, x in [0, 1]
+__device__ __forceinline__ void w_sincospi(float x, float *s, float *c) { w_sincos_rev(0.5f * x, s, c); }
+__device__ __forceinline__ void w_sincospi(double x, double *s, double *c) { sincospi(x, s, c); }
 // sin and cos of a small angle x in radians
 __device__ __forceinline__ void w_sincos_small(float x, float *s, float *c) {
   w_sincos_rev(x * 0.159154943091895336f, s, c);
@@ -359,19 +362,19 @@ __device__ __forceinline__ void scatter2(Photon<R> &p, R g, S &rs, const WalkPar
   const R xi = Unif<R>::halfup(rs.next(P));
   R s, c;
   if (g != R(0)) {
-    m_sincospi(xi, &s, &c);
-    const R q = (R(1) - g) / (R(1) + g);
+    w_sincospi(xi, &s, &c);
+    const R q = w_div(R(1) - g, R(1) + g);
     const R a = q * s;
-    const R den = R(1) / (c * c + a * a);
+    const R den = w_div(R(1), c * c + a * a);
     const R ct = (c * c - a * a) * den, st = R(2) * a * c * den;
     // the reference rebuilds the direction from its angle, so the result
     // is a unit vector whatever the input length
-    const R il = R(1) / m_sqrt(p.d[0] * p.d[0] + p.d[1] * p.d[1]);
+    const R il = w_rsqrt(p.d[0] * p.d[0] + p.d[1] * p.d[1]);
     const R dx = p.d[0] * il, dy = p.d[1] * il;
     p.d[0] = dx * ct - dy * st;
     p.d[1] = dx * st + dy * ct;
   } else {
-    m_sincospi(R(2) * xi, &s, &c);
+    w_sincospi(R(2) * xi, &s, &c);
     p.d[0] = c;
     p.d[1] = s;
   }
