@@ -287,21 +287,21 @@ __device__ __forceinline__ void load_elem(EP E, int el,
 template <int D, class R, class S>
 __device__ __forceinline__ bool fresnel(Photon<R> &p, const R n[3], R n1, R n2, bool refract,
                                         S &rs, const WalkParams &P) {
-  const R eta = n1 / n2;
+  const R eta = w_div(n1, n2);
   R ci = p.d[0] * n[0] + p.d[1] * n[1];
   if (D == 3) ci += p.d[2] * n[2];
   const R s2t = eta * eta * (R(1) - ci * ci);
   bool refl = true;
   if (s2t < R(1)) {
-    const R ct = m_sqrt(R(1) - s2t);
-    const R rs_ = (eta * ci - ct) / (eta * ci + ct);
-    const R rp = (ci - eta * ct) / (ci + eta * ct);
+    const R ct = w_sqrt(R(1) - s2t);   // 1 - s2t >= 2^-24 in FP32: a normal number
+    const R rs_ = w_div(eta * ci - ct, eta * ci + ct);
+    const R rp = w_div(ci - eta * ct, ci + eta * ct);
     R Rf = R(0.5) * (rs_ * rs_ + rp * rp);
     // a direction longer than unit length (an unnormalised absolute light
     // direction before its first scattering) makes the reference's
     // acos(costhi) NaN, and its sin(thi + tht) > eps test then selects the
     // normal-incidence value (MC3D.hpp:1409-1419)
-    if (ci > R(1)) Rf = ((eta - R(1)) / (eta + R(1))) * ((eta - R(1)) / (eta + R(1)));
+    if (ci > R(1)) Rf = w_div(eta - R(1), eta + R(1)) * w_div(eta - R(1), eta + R(1));
     refl = Unif<R>::closed(rs.next(P)) <= Rf;
     if (!refl && refract) {
       const R s = ct - eta * ci;
@@ -849,7 +849,7 @@ __device__ __forceinline__ void walk_body(const WalkParams &P) {
               kf = R(it.kfac);
             }
             create_photon<D, R>(p, rs, it, P);
-            p.prop = -m_log(Unif<R>::open(rs.next(P))) / (SMALL ? EL[p.el].opt.y : E[p.el].opt.y);
+            p.prop = w_div(w_neglog(Unif<R>::open(rs.next(P))), SMALL ? EL[p.el].opt.y : E[p.el].opt.y);
             alive = true;
           }
         }
@@ -1125,7 +1125,7 @@ __device__ __forceinline__ void walk_body(const WalkParams &P) {
       const R mus_c = mus, n_c = nidx;
       if (mus_c <= R(0) && mus_n > R(0)) {
         // vacuum -> scattering medium: fresh free path
-        p.prop = w_neglog(Unif<R>::open(rs.next(P))) / mus_n;
+        p.prop = w_div(w_neglog(Unif<R>::open(rs.next(P))), mus_n);
       }
       // the roulette constants come as FP32 copies for the FP32 walk, so
       // they stay in scalar registers
@@ -1139,7 +1139,7 @@ __device__ __forceinline__ void walk_body(const WalkParams &P) {
           alive = false;
           return;
         }
-        const R wn = p.w / chance;
+        const R wn = w_div(p.w, chance);
         led_add(3, (double)wn - (double)p.w);
         p.w = wn;
       }
