@@ -85,7 +85,8 @@ typedef struct vmc_run_desc {
   uint64_t seed;         /* photon i of a run draws from the stream (seed, i)  */
   int32_t precision;     /* VMC_FP64 | VMC_FP32                                */
   int32_t tally;         /* VMC_TALLY_FP64 | VMC_TALLY_FIXED64                 */
-  int32_t block_size;    /* 0 = default (FP32 1024, FP64 256); multiple of 64   */
+  int32_t block_size;    /* 0 = default (FP32 1024, or 512 for a launch of
+                            several items; FP64 256); multiple of 64     */
   int32_t blocks_per_cu; /* resident workgroups per CU: 0 = as many as fit,
                             else at most this many                          */
   int32_t replicas;      /* tally copies: > 0 that many (device-scope atomics);

@@ -320,6 +320,13 @@ void server_loop(vmc_server *s) {
           slice.push_back(n);
         }
         group.resize(items.size());   // the shortest policy may leave requests out
+        if (items.size() == 1) {
+          // one request walks alone: the single-item kernel's launch shape
+          if (resolve_cfg(gdim, s->num_cus, s->lds_per_cu, gfd, &tmpl, &c, false) != VMC_OK)
+            throw HipFail{hipErrorInvalidValue, "server launch configuration"};
+          c.weight0 = group[0]->model.weight0;
+          c.chance = group[0]->model.chance;
+        }
         a.ne = s->cap_words / c.W;
         a.nb = s->cap_b;
         a.nitems = items.size();
