@@ -118,7 +118,8 @@ def copy_reference(nbytes: int):
 
 
 TRAFFIC_PROFILES = {"box": "walk3d_fp32_box384k_hbm_traffic.txt", "fd": "walk3d_fp32fd_box2m_hbm_traffic.txt"}
-FLOP_PROFILES = {"box": "walk3d_fp32_box384k_flops.txt", "fd": "walk3d_fp32fd_box2m_flops.txt"}
+FLOP_PROFILES = {"box": "walk3d_fp32_box384k_flops.txt", "fd": "walk3d_fp32fd_box2m_flops.txt",
+                 "square2d": "walk2d_fp32_square200_flops.txt"}
 
 
 def _profile_value(name, key):
@@ -515,7 +516,7 @@ def main():
                 "steps_per_launch": steps_total // a.steps,
                 "kernel_ms_per_launch": round(kernel_s * 1e3 / a.steps, 3)}
 
-    fps = flops_per_step(a.workload) if (p.dim == 3 and a.precision == "fp32") else None
+    fps = flops_per_step(a.workload) if a.precision == "fp32" else None
     flop_step = fps if fps is not None else FLOP_STEP_ESTIMATE
     vpeak = FP64_PEAK_TFLOPS if a.precision == "fp64" else FP32_PEAK_TFLOPS
     vector = {"achieved": round(steps_total * flop_step / kernel_s / 1e12, 3), "peak": vpeak,
