@@ -73,6 +73,8 @@ def parse():
                     help="serve: mesh sizes of the serving pool times this (1.0: 1k-200k triangles, 50k-2M tets)")
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="default line's batch part: photon counts times this (0: leave the part out)")
+    ap.add_argument("--serve-launch-photons", type=float, default=0,
+                    help="serve: photons per server launch (0: the server's default)")
     ap.add_argument("--policy", choices=("fair", "shortest"), default="shortest",
                     help="serve workload: the server's scheduling policy")
     ap.add_argument("--utilization", type=float, default=0.8,
@@ -356,6 +358,7 @@ def serve_leg(a, nreq: int, ncal: int):
     reqs = meshgen.serving_requests(nreq, seed=2025)
     cache = serving.MeshCache(budget_bytes=16 << 30)
     kw = dict(precision=vmc.FP32, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64)
+    skw = dict(kw, launch_photons=int(a.serve_launch_photons))
     policy = vmc.SERVE_SHORTEST if a.policy == "shortest" else vmc.SERVE_FAIR
     print(f"serve: service rates of the {len(pool)} meshes", file=sys.stderr, flush=True)
     # service rate per (mesh, mode) from short single runs
@@ -375,7 +378,7 @@ def serve_leg(a, nreq: int, ncal: int):
     ncal = min(len(reqs), ncal)
     cal = [dict(q, seed=q["seed"] + 100000) for q in reqs[:ncal]]
     print(f"serve: calibration stream of {ncal} requests at {lam0:.2f}/s", file=sys.stderr, flush=True)
-    with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
+    with vmc.Server(arena_bytes=16 << 30, policy=policy, **skw) as srv:
         _, wcal = serving.drive(srv, cache, pool, cal, lam0, seed=11)
         busy0 = srv.counters()["busy_ms"] / 1e3 / wcal
     # the calibration requests' device time in the mix against their alone
@@ -386,7 +389,7 @@ def serve_leg(a, nreq: int, ncal: int):
     keep = list(range(0, len(reqs), max(1, len(reqs) // 4)))[:4]
     print(f"serve: calibration busy {busy0:.3f}, sharing {share:.3f}; {len(reqs)} requests at {lam:.2f}/s",
           file=sys.stderr, flush=True)
-    with vmc.Server(arena_bytes=16 << 30, policy=policy, **kw) as srv:
+    with vmc.Server(arena_bytes=16 << 30, policy=policy, **skw) as srv:
         out, wall = serving.drive(srv, cache, pool, reqs, lam, seed=7, keep=keep)
         ctr = srv.counters()
     lat = np.array([o["latency_ms"] for o in out])
