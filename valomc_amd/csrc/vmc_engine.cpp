@@ -350,8 +350,9 @@ std::mutex g_batch_mutex;
 // +inf when the device refuses), and a knob moves only for a gain above 2 %.
 // The first timing of the base configuration only warms up (arena
 // allocation, code loading).
+// (several: a batch of several items, whose FP32 default block is 512)
 template <class TimeOf>
-int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trials, TimeOf time_of) {
+int autotune_knobs(const vmc_run_desc &base, bool several, vmc_run_desc *tuned, int32_t *trials, TimeOf time_of) {
   int count = 0;
   auto timed = [&](const vmc_run_desc &c) {
     ++count;
@@ -360,7 +361,7 @@ int autotune_knobs(const vmc_run_desc &base, vmc_run_desc *tuned, int32_t *trial
   vmc_run_desc best = base;
   const bool f64 = base.precision == VMC_FP64;
   // defaults spelled out, so a knob's default value is not timed twice
-  if (best.block_size == 0) best.block_size = f64 ? 256 : 1024;
+  if (best.block_size == 0) best.block_size = f64 ? 256 : (several ? 512 : 1024);
   if (best.refill_min == 0) best.refill_min = default_refill(2.0 * M_PI * base.f > 0.0);
   if (best.chunk == 0) best.chunk = (int32_t)kChunk;
   if (!std::isfinite(timed(best))) return VMC_ERR_INVALID_ARGUMENT;
@@ -870,7 +871,7 @@ int vmc_problem_autotune(vmc_problem *m, const vmc_run_desc *run, vmc_run_desc *
   const int64_t n = trial_photons > 0 ? trial_photons
                                       : std::min<int64_t>(10000000, std::max<int64_t>(100000, run->Nphoton / 50));
   vmc_output none{};
-  const int st = autotune_knobs(*run, tuned, trials, [&](vmc_run_desc cand) {
+  const int st = autotune_knobs(*run, false, tuned, trials, [&](vmc_run_desc cand) {
     cand.Nphoton = n;
     vmc_stats s{};
     return vmc_problem_run(m, &cand, &none, &s, nullptr, nullptr) == VMC_OK ? s.kernel_ms
@@ -894,7 +895,7 @@ int vmc_batch_autotune(const vmc_batch_item *items, int32_t n, double trial_frac
     if (b.run.Nphoton > 0) b.run.Nphoton = std::max<int64_t>(1, (int64_t)(frac * (double)b.run.Nphoton));
     b.out = vmc_output{};
   }
-  return autotune_knobs(items[0].run, tuned, trials, [&](vmc_run_desc cand) {
+  return autotune_knobs(items[0].run, n > 1, tuned, trials, [&](vmc_run_desc cand) {
     // the candidate's launch knobs on every item; each item keeps its own
     // photons, stream, phase and model constants
     for (vmc_batch_item &b : trial) {
