@@ -485,13 +485,16 @@ def main():
               replicas=a.replicas, tally=vmc.TALLY_FIXED64 if a.tally == "fixed" else vmc.TALLY_FP64,
               f=float(getattr(p, "f", 0.0) or 0.0))
 
+    prec = vmc.FP64 if a.precision == "fp64" else vmc.FP32
+    # the FP64 comparison run keeps the FP64 launch defaults (FP32 launch
+    # knobs, such as 1024-lane workgroups, are not valid for it)
+    kw64 = dict(tally=kw["tally"], f=kw["f"])
     tuned = None
     if a.autotune:
         t = time.perf_counter()
-        tuned = prob.autotune(nphot, precision=vmc.FP32, **kw)
+        tuned = prob.autotune(nphot, precision=prec, **kw)
         tuned["tuning_s"] = round(time.perf_counter() - t, 3)
         kw.update({k: tuned[k] for k in ("block_size", "refill_min", "chunk", "replicas", "lds_slots", "cache_epoch")})
-    prec = vmc.FP64 if a.precision == "fp64" else vmc.FP32
     for i in range(a.warmup):
         prob.run(nphot, seed=100 + i, precision=prec, **kw)
     runs = []
@@ -532,7 +535,7 @@ def main():
     if not a.no_fp64 and a.precision == "fp32":
         f32 = runs[0]
         t = time.perf_counter()
-        f64 = prob.run(nphot, seed=1, precision=vmc.FP64, **kw)
+        f64 = prob.run(nphot, seed=1, precision=vmc.FP64, **kw64)
         t64 = time.perf_counter() - t
         rel = float(np.linalg.norm(f32.ER - f64.ER) / np.linalg.norm(f64.ER))
         extra = {"fp64_photons_per_s": round(nphot / t64, 1),
