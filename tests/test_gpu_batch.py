@@ -196,6 +196,37 @@ def test_many_items_equal_single_runs(vmc):
         probq.close()
 
 
+@pytest.mark.parametrize("epoch", [1, 3, 64])
+def test_batch_cache_epochs_keep_results(vmc, epoch):
+    """A batch's cache epochs (all waves meet at a barrier, move the slots'
+    sums to the replica and free the slots) at short intervals, so they run
+    many times while items start and end: every item still equals its solve
+    alone bit for bit (fixed point), CW and frequency domain, and its ledger
+    closes."""
+    p = _golden("p3_pencil_sphere")
+    q = _golden("p3_direct_abs_fd")
+    counts = [40000, 0, 1, 129] + [2000 + 900 * k for k in range(12)]
+    try:
+        prob, probq = vmc.Problem(p), vmc.Problem(q)
+        items = [(prob, dict(nphoton=n, seed=31, stream_tag=k)) for k, n in enumerate(counts)]
+        items += [(probq, dict(nphoton=n, seed=7, stream_tag=k, f=q.f, phase0=q.phase0))
+                  for k, n in enumerate(counts[4:])]
+        print(f"batch of {len(items)} items, cache epoch every {epoch} passes", flush=True)
+        res, stats = vmc.batch_run(items, precision=vmc.FP32, tally=vmc.TALLY_FIXED64, cache_epoch=epoch)
+        assert stats.photons == sum(kw["nphoton"] for _, kw in items)
+        for (pr, kw), r in zip(items, res):
+            kw = dict(kw)
+            n = kw.pop("nphoton")
+            alone = pr.run(n, precision=vmc.FP32, tally=vmc.TALLY_FIXED64, **kw)
+            for k in OUT:
+                assert np.array_equal(getattr(r, k), getattr(alone, k), equal_nan=True), (kw, k)
+            assert r.loss == alone.loss
+            _check_ledger(r, alone, n, 1e-6)
+    finally:
+        prob.close()
+        probq.close()
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_batch_plan_replays_equal_eager_batches(vmc, precision):
     """A batch recorded once as a HIP graph (2D and 3D, CW and FD groups, an
