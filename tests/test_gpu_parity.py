@@ -277,6 +277,30 @@ def test_mesh_residency_follows_size(vmc):
         assert not b.run(20000, precision=vmc.FP64).lds_mesh
 
 
+def test_mesh_residency_around_the_lds_limit(vmc):
+    """Square grids from 288 to 1352 triangles cross the LDS limits of the
+    small-mesh path (about 950 element slots in FP32, 320 in FP64 CW and
+    224 in FP64 frequency domain): every size runs, the largest ones from
+    device memory, and each equals the general path bit for bit in fixed
+    point."""
+    from valomc_amd import meshgen
+    seen = set()
+    for cells in (12, 16, 20, 22, 24, 26):
+        r, H, BH = meshgen.rectangular_mesh(10, 10, 10.0 / cells)
+        p = meshgen.homogeneous(2, r, H, BH, 0.01, 1.0, 0.9, 1.3, 1.0)
+        p.BCType[:cells // 2] = ord("c")
+        with vmc.Problem(p) as prob:
+            for prec in (vmc.FP32, vmc.FP64):
+                for f in (0.0, 100e6):
+                    run = lambda **kw: prob.run(20000, seed=5, f=f, precision=prec, tally=vmc.TALLY_FIXED64, **kw)
+                    a, b = run(), run(lds_slots=-1)
+                    print(f"{2 * cells * cells} triangles, prec {prec}, f {f}: lds_mesh {a.lds_mesh}", flush=True)
+                    seen.add((prec, a.lds_mesh))
+                    for k in OUT:
+                        assert np.array_equal(getattr(a, k), getattr(b, k), equal_nan=True), (cells, prec, f, k)
+    assert seen == {(vmc.FP32, True), (vmc.FP32, False), (vmc.FP64, True), (vmc.FP64, False)}
+
+
 def test_tally_flush_mode_is_validated(vmc):
     """direct_flush takes 0 or 1 only."""
     p = _golden("p3_pencil_sphere")
